@@ -1,0 +1,760 @@
+// C-ABI of the tonal path: context, staging of cases into HBM, tile planning, the persistent
+// eval launch, the ordered reduction, timing, graph replay and the autotuner.
+// Replaces the device-side part of the reference driver, main.cu:98-247.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+#include "abi_internal.hpp"
+
+namespace acum {
+hipError_t launch_tonal_eval(bool mixed, int W, int grid, hipStream_t s, const TonalCaseDev* cases,
+                             const Tile* tiles, const Item* items, int nItems, int* counter,
+                             uint32_t* flags);
+hipError_t eval_occupancy(bool mixed, int W, int* blocks_per_cu);
+hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* cases, int icase,
+                               int nT, int nObs);
+hipError_t launch_tonal_prep(bool mixed, hipStream_t s, const float* X, const float* N,
+                             const float* V, const float* PR, int64_t n, double inv_a0, void* rec);
+hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes);
+hipError_t launch_reset(hipStream_t s, int* counter, uint32_t* flags);
+hipError_t launch_tonal_bins(bool mixed, hipStream_t s, const void* rec, int64_t nE,
+                             const TimeConst& k, double ox, double oy, double oz, int32_t* ipts,
+                             int32_t* npi, int32_t* nd, int32_t* nu);
+}  // namespace acum
+
+using namespace acum;
+
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct StagedCase {
+  int64_t nE = 0, nT = 0, nObs = 0;
+  TimeConst k{};
+  double rhoRef = 0, D1 = 0, Minf = 0;
+  std::vector<int64_t> surf0;  // first element of each surface
+  std::vector<float> pos;      // host positions [nE][nT][3], for planning
+  float* d_obs = nullptr;
+  float* d_aos[4] = {nullptr, nullptr, nullptr, nullptr};  // X, N, V, PR
+  void* d_rec[2] = {nullptr, nullptr};                      // [ACUM_FP64], [ACUM_MIXED]
+  float* d_out[3] = {nullptr, nullptr, nullptr};
+};
+
+struct Plan {
+  bool valid = false;
+  acum_tune tune{};
+  std::vector<Tile> tiles;
+  std::vector<Item> items;
+  std::vector<int> G;
+  std::vector<int64_t> slab_off;
+  int64_t slab_bytes = 0;
+  Tile* d_tiles = nullptr;
+  Item* d_items = nullptr;
+  TonalCaseDev* d_cases = nullptr;
+  void* d_slab = nullptr;
+  int grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+struct Stage {
+  std::vector<StagedCase> cases;
+  Plan plan[2];
+};
+
+}  // namespace
+
+struct acum_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  hipEvent_t ev[6] = {};
+  acum_timing timing{};
+  acum_tune tune[2]{};
+  int* d_counter = nullptr;
+  uint32_t* d_flags = nullptr;
+  int cu_count = 0;
+  std::map<int, std::unique_ptr<Stage>> stages;
+  int next_handle = 1;
+};
+
+namespace acum {
+
+int ctx_fail(acum_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+hipStream_t ctx_stream(acum_ctx* c) { return c->stream; }
+hipEvent_t ctx_event(acum_ctx* c, int i) { return c->ev[i]; }
+acum_timing* ctx_timing(acum_ctx* c) { return &c->timing; }
+int ctx_device(acum_ctx* c) { return c->device; }
+
+}  // namespace acum
+
+#define HIPCHK(ctx, call)                                                                  \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return ctx_fail(ctx, e_ == hipErrorOutOfMemory ? ACUM_ERR_NOMEM : ACUM_ERR_DEVICE,   \
+                      std::string(#call) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+namespace {
+
+void free_plan(Plan& p) {
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  (void)hipFree(p.d_tiles);
+  (void)hipFree(p.d_items);
+  (void)hipFree(p.d_cases);
+  (void)hipFree(p.d_slab);
+  p = Plan();
+}
+
+void free_stage(Stage& s) {
+  for (auto& c : s.cases) {
+    (void)hipFree(c.d_obs);
+    for (auto& p : c.d_aos) (void)hipFree(p);
+    for (auto& p : c.d_rec) (void)hipFree(p);
+    for (auto& p : c.d_out) (void)hipFree(p);
+  }
+  for (auto& p : s.plan) free_plan(p);
+}
+
+TimeConst make_time_const(const acum_tonal_case* c) {
+  // The CPU oracle builds the same constants with the same operations.
+  TimeConst k{};
+  k.a0 = c->a0;
+  k.inv_a0 = 1.0 / c->a0;
+  k.dTau = c->dTau;
+  k.oM = c->oM;
+  k.nT = (int32_t)c->nTime;
+  k.dPsi = c->dPsi;
+  k.inv_dPsi = 1.0 / c->dPsi;
+  k.lpi = (double)c->nTime * c->dPsi;
+  k.inv_lpi = 1.0 / k.lpi;
+  k.vx = c->vInf[0];
+  k.vy = c->vInf[1];
+  k.vz = c->vInf[2];
+  double vv = k.vx * k.vx;
+  vv = vv + k.vy * k.vy;
+  vv = vv + k.vz * k.vz;
+  const double a = vv - k.a0 * k.a0;
+  k.four_a = 4.0 * a;
+  k.inv_2a = 1.0 / (2.0 * a);
+  k.still = (k.vx == 0.0 && k.vy == 0.0 && k.vz == 0.0) ? 1 : 0;
+  return k;
+}
+
+int validate(acum_ctx* ctx, const acum_tonal_case* c) {
+  if (!c) return ctx_fail(ctx, ACUM_ERR_ARG, "null case");
+  if (c->nSurf < 1 || c->nTime < 4 || c->nObs < 1)
+    return ctx_fail(ctx, ACUM_ERR_ARG, "need nSurf >= 1, nTime >= 4, nObs >= 1");
+  if (c->nTime > (1 << 20) || c->nObs > (1 << 26))
+    return ctx_fail(ctx, ACUM_ERR_ARG, "nTime or nObs too large");
+  if (!c->E || !c->X || !c->N || !c->V || !c->PR || !c->XPtr || !c->NPtr || !c->VPtr ||
+      !c->PRPtr || !c->rObs)
+    return ctx_fail(ctx, ACUM_ERR_ARG, "null array in case");
+  for (int64_t s = 0; s < c->nSurf; s++)
+    if (c->E[s] < 1) return ctx_fail(ctx, ACUM_ERR_ARG, "surface with no elements");
+  if (!(c->a0 > 0) || !(c->dPsi > 0) || !(c->dTau > 0) || !(c->oM > 0))
+    return ctx_fail(ctx, ACUM_ERR_ARG, "a0, dPsi, dTau and oM must be positive");
+  const double vv = c->vInf[0] * c->vInf[0] + c->vInf[1] * c->vInf[1] + c->vInf[2] * c->vInf[2];
+  if (!(std::sqrt(vv) < c->a0))
+    return ctx_fail(ctx, ACUM_ERR_NUMERIC, "free-stream Mach >= 1: retarded time undefined");
+  if (c->precision != ACUM_FP64 && c->precision != ACUM_MIXED)
+    return ctx_fail(ctx, ACUM_ERR_ARG, "precision must be ACUM_FP64 or ACUM_MIXED");
+  return ACUM_OK;
+}
+
+int stage_case(acum_ctx* ctx, const acum_tonal_case* c, StagedCase& sc) {
+  int rc = validate(ctx, c);
+  if (rc) return rc;
+  const int64_t nT = c->nTime;
+  sc.nT = nT;
+  sc.nObs = c->nObs;
+  sc.k = make_time_const(c);
+  sc.rhoRef = c->rhoRef;
+  sc.D1 = 1.0 / (4.0 * M_PI * c->a0);
+  sc.Minf = std::sqrt(c->vInf[0] * c->vInf[0] + c->vInf[1] * c->vInf[1] + c->vInf[2] * c->vInf[2]) / c->a0;
+  sc.surf0.resize(c->nSurf);
+  int64_t nE = 0;
+  for (int64_t s = 0; s < c->nSurf; s++) {
+    sc.surf0[s] = nE;
+    nE += c->E[s];
+  }
+  sc.nE = nE;
+  const int64_t q = nE * nT;
+  const int width[4] = {4, 6, 6, 2};
+  const float* src[4] = {c->X, c->N, c->V, c->PR};
+  const int64_t* off[4] = {c->XPtr, c->NPtr, c->VPtr, c->PRPtr};
+  for (int a = 0; a < 4; a++) {
+    HIPCHK(ctx, hipMalloc(&sc.d_aos[a], (size_t)q * width[a] * sizeof(float)));
+    int64_t e0 = 0;
+    for (int64_t s = 0; s < c->nSurf; s++) {
+      const size_t n = (size_t)c->E[s] * nT * width[a];
+      HIPCHK(ctx, hipMemcpyAsync(sc.d_aos[a] + e0 * nT * width[a], src[a] + off[a][s],
+                                 n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+      e0 += c->E[s];
+    }
+  }
+  sc.pos.resize((size_t)q * 3);
+  {
+    int64_t e0 = 0;
+    for (int64_t s = 0; s < c->nSurf; s++) {
+      const float* X = c->X + c->XPtr[s];
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < c->E[s] * nT; i++)
+        for (int d = 0; d < 3; d++) sc.pos[(e0 * nT + i) * 3 + d] = X[i * 4 + d];
+      e0 += c->E[s];
+    }
+  }
+  HIPCHK(ctx, hipMalloc(&sc.d_obs, (size_t)c->nObs * 3 * sizeof(float)));
+  HIPCHK(ctx, hipMemcpyAsync(sc.d_obs, c->rObs, (size_t)c->nObs * 3 * sizeof(float),
+                             hipMemcpyHostToDevice, ctx->stream));
+  for (int i = 0; i < 3; i++) HIPCHK(ctx, hipMalloc(&sc.d_out[i], (size_t)nT * c->nObs * sizeof(float)));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ACUM_OK;
+}
+
+int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
+  if (sc.d_rec[prec]) return ACUM_OK;
+  if (!sc.d_aos[0]) return ctx_fail(ctx, ACUM_ERR_ARG, "staged element data already released");
+  const bool mixed = prec == ACUM_MIXED;
+  const size_t bytes = (size_t)sc.nE * sc.nT * kRecFields * (mixed ? sizeof(float) : sizeof(double));
+  HIPCHK(ctx, hipMalloc(&sc.d_rec[prec], bytes));
+  HIPCHK(ctx, launch_tonal_prep(mixed, ctx->stream, sc.d_aos[0], sc.d_aos[1], sc.d_aos[2], sc.d_aos[3],
+                                sc.nE * sc.nT, sc.k.inv_a0, sc.d_rec[prec]));
+  if (sc.d_rec[0] && sc.d_rec[1]) {  // both record sets built: the AoS copy is no longer needed
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (auto& p : sc.d_aos) {
+      (void)hipFree(p);
+      p = nullptr;
+    }
+  }
+  return ACUM_OK;
+}
+
+// Tiles of one case: runs of consecutive elements of one surface whose positions stay within
+// `max_spread` observer-time bins of the first element's at every step (Lipschitz bound of the
+// convected retarded distance: |dr| <= |dx| / (1 - Minf)), with the pass length K chosen so the
+// bins of a pass fit the window:  (K + 1) * step_bins + 2 * spread + 6 <= W.
+int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vector<Tile>& tiles) {
+  const int64_t nT = sc.nT;
+  const double L = sc.k.oM / (sc.k.a0 * (1.0 - sc.Minf) * sc.k.dPsi);  // bins per metre
+  const int W = t.window;
+  const float* P = sc.pos.data();
+  auto dist2 = [&](int64_t e1, int64_t k1, int64_t e2, int64_t k2) {
+    const float* a = P + (e1 * nT + k1) * 3;
+    const float* b = P + (e2 * nT + k2) * 3;
+    const double dx = (double)a[0] - b[0], dy = (double)a[1] - b[1], dz = (double)a[2] - b[2];
+    return dx * dx + dy * dy + dz * dz;
+  };
+  std::vector<double> step(sc.nE);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < sc.nE; e++) {
+    double m = 0;
+    for (int64_t k = 0; k < nT; k++) m = std::max(m, dist2(e, k, e, (k + 1) % nT));
+    step[e] = std::sqrt(m);
+  }
+  auto pass_len = [&](int spread, double S) {
+    const double sb = 1.0 + S * L;
+    const int K = (int)std::floor((W - 2.0 * spread - 6.0) / sb) - 1;
+    return std::min<int64_t>(std::min<int64_t>(K, t.max_chunk), nT);
+  };
+  const size_t nsurf = sc.surf0.size();
+  std::vector<std::vector<Tile>> per(nsurf);
+  int status = ACUM_OK;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t s = 0; s < nsurf; s++) {
+    const int64_t a = sc.surf0[s], b = (s + 1 < nsurf) ? sc.surf0[s + 1] : sc.nE;
+    int64_t e0 = a;
+    while (e0 < b) {
+      double R = 0, S = step[e0];
+      int spread = 1;
+      int K = (int)pass_len(spread, S);
+      if (K < 1) {
+#pragma omp critical
+        status = ACUM_ERR_PLAN;
+        break;
+      }
+      int64_t e = e0 + 1;
+      for (; e < b && e - e0 < t.max_tile; e++) {
+        double d = 0;
+        for (int64_t k = 0; k < nT; k++) d = std::max(d, dist2(e, k, e0, k));
+        const double Rn = std::max(R, std::sqrt(d)), Sn = std::max(S, step[e]);
+        const int sp = (int)std::ceil(Rn * L) + 1;
+        const int Kn = (int)pass_len(sp, Sn);
+        if (sp > t.max_spread || Kn < 4) break;
+        R = Rn;
+        S = Sn;
+        spread = sp;
+        K = Kn;
+      }
+      Tile tl{};
+      tl.e0 = e0;
+      tl.ne = (int32_t)(e - e0);
+      tl.spread = spread;
+      tl.K = K;
+      per[s].push_back(tl);
+      e0 = e;
+    }
+  }
+  if (status) return ctx_fail(ctx, ACUM_ERR_PLAN, "observer-time window too small for the source speed");
+  tiles.clear();
+  for (auto& v : per) tiles.insert(tiles.end(), v.begin(), v.end());
+  return ACUM_OK;
+}
+
+int build_plan(acum_ctx* ctx, Stage& st, int prec) {
+  Plan& p = st.plan[prec];
+  if (p.valid && std::memcmp(&p.tune, &ctx->tune[prec], sizeof(acum_tune)) == 0) return ACUM_OK;
+  free_plan(p);
+  p.tune = ctx->tune[prec];
+  const bool mixed = prec == ACUM_MIXED;
+  int bpc = 0;
+  HIPCHK(ctx, eval_occupancy(mixed, p.tune.window, &bpc));
+  if (bpc < 1) return ctx_fail(ctx, ACUM_ERR_DEVICE, "eval kernel does not fit a compute unit");
+  const int capacity = bpc * ctx->cu_count;
+  const int nc = (int)st.cases.size();
+  std::vector<std::vector<Tile>> ct(nc);
+  for (int c = 0; c < nc; c++) {
+    int rc = plan_tiles(ctx, st.cases[c], p.tune, ct[c]);
+    if (rc) return rc;
+  }
+  const double target_items = 4.0 * capacity;
+  const size_t asz = mixed ? sizeof(float) : sizeof(double);
+  p.G.assign(nc, 1);
+  p.slab_off.assign(nc, 0);
+  p.slab_bytes = 0;
+  for (int c = 0; c < nc; c++) {
+    const StagedCase& sc = st.cases[c];
+    const int nog = (int)((sc.nObs + 63) / 64);
+    // G depends on this case alone, so a case gets the same tiles, groups and summation order
+    // whether it runs alone or inside a batch (bit-identical results).
+    int G = p.tune.groups > 0 ? p.tune.groups : (int)std::lround(target_items / nog);
+    G = std::max(1, std::min<int>(G, (int)ct[c].size()));
+    const int64_t row = sc.nObs * sc.nT * 2 * (int64_t)asz;
+    const int64_t cap = (int64_t)8 << 30;  // keep each case's partials under 8 GiB
+    while (G > 1 && G * row > cap) G--;
+    p.G[c] = G;
+    p.slab_off[c] = p.slab_bytes;
+    p.slab_bytes += ((G * row + 255) / 256) * 256;
+    // split the tiles into G runs of balanced element count
+    int64_t tot = 0;
+    for (auto& tl : ct[c]) tot += tl.ne;
+    const int base = (int)p.tiles.size();
+    p.tiles.insert(p.tiles.end(), ct[c].begin(), ct[c].end());
+    const int nt = (int)ct[c].size();
+    int t0 = 0;
+    int64_t acc = 0;
+    for (int g = 0; g < G; g++) {
+      const int64_t goal = tot * (g + 1) / G;
+      int t1 = t0;
+      int64_t grp = 0;
+      // at least one tile per group; stop before overshooting the goal; leave one tile for
+      // each later group; the last group takes the rest
+      while (t1 < nt && (t1 == t0 || g == G - 1 ||
+                         (acc + ct[c][t1].ne <= goal && nt - (t1 + 1) >= G - 1 - g))) {
+        acc += ct[c][t1].ne;
+        grp += ct[c][t1].ne;
+        t1++;
+      }
+      for (int og = 0; og < nog; og++) {
+        Item it{};
+        it.icase = c;
+        it.obs0 = og * 64;
+        it.tile0 = base + t0;
+        it.ntile = t1 - t0;
+        it.group = g;
+        it.cost = grp * sc.nT * 64;
+        if (it.ntile > 0) p.items.push_back(it);
+      }
+      t0 = t1;
+    }
+  }
+  std::stable_sort(p.items.begin(), p.items.end(),
+                   [](const Item& a, const Item& b) { return a.cost > b.cost; });
+  HIPCHK(ctx, hipMalloc(&p.d_tiles, p.tiles.size() * sizeof(Tile)));
+  HIPCHK(ctx, hipMalloc(&p.d_items, p.items.size() * sizeof(Item)));
+  HIPCHK(ctx, hipMalloc(&p.d_cases, nc * sizeof(TonalCaseDev)));
+  HIPCHK(ctx, hipMalloc(&p.d_slab, p.slab_bytes));
+  std::vector<TonalCaseDev> hc(nc);
+  for (int c = 0; c < nc; c++) {
+    const StagedCase& sc = st.cases[c];
+    TonalCaseDev& d = hc[c];
+    d.rec = sc.d_rec[prec];
+    d.obs = sc.d_obs;
+    d.slab = (char*)p.d_slab + p.slab_off[c];
+    d.pT = sc.d_out[0];
+    d.pL = sc.d_out[1];
+    d.pA = sc.d_out[2];
+    d.nE = sc.nE;
+    d.nObs = (int32_t)sc.nObs;
+    d.G = p.G[c];
+    d.k = sc.k;
+    d.rhoRef = sc.rhoRef;
+    d.D1 = sc.D1;
+  }
+  HIPCHK(ctx, hipMemcpyAsync(p.d_tiles, p.tiles.data(), p.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(p.d_items, p.items.data(), p.items.size() * sizeof(Item), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(p.d_cases, hc.data(), nc * sizeof(TonalCaseDev), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  p.grid = std::min<int>((int)p.items.size(), capacity);
+  p.valid = true;
+  return ACUM_OK;
+}
+
+// Enqueues one full tonal pass: reset, zero partials, eval, reductions.
+int enqueue_pass(acum_ctx* ctx, Stage& st, int prec, bool with_events) {
+  Plan& p = st.plan[prec];
+  const bool mixed = prec == ACUM_MIXED;
+  hipStream_t s = ctx->stream;
+  if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
+  HIPCHK(ctx, launch_reset(s, ctx->d_counter, ctx->d_flags));
+  HIPCHK(ctx, launch_fill_zero(s, p.d_slab, p.slab_bytes));
+  if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
+  HIPCHK(ctx, launch_tonal_eval(mixed, p.tune.window, p.grid, s, p.d_cases, p.d_tiles, p.d_items,
+                                (int)p.items.size(), ctx->d_counter, ctx->d_flags));
+  if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
+  for (int c = 0; c < (int)st.cases.size(); c++)
+    HIPCHK(ctx, launch_tonal_reduce(mixed, s, p.d_cases, c, (int)st.cases[c].nT, (int)st.cases[c].nObs));
+  if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[3], s));
+  return ACUM_OK;
+}
+
+int check_flags(acum_ctx* ctx) {
+  uint32_t f = 0;
+  HIPCHK(ctx, hipMemcpy(&f, ctx->d_flags, sizeof(f), hipMemcpyDeviceToHost));
+  if (f & kFlagWindow)
+    return ctx_fail(ctx, ACUM_ERR_PLAN, "a contribution fell outside its tile window");
+  if (f & (kFlagNumeric | kFlagSweep))
+    return ctx_fail(ctx, ACUM_ERR_NUMERIC,
+                    "non-finite observer time, source on an observer, or |Mr| >= 1");
+  return ACUM_OK;
+}
+
+Stage* get_stage(acum_ctx* ctx, int32_t handle) {
+  auto it = ctx->stages.find(handle);
+  return it == ctx->stages.end() ? nullptr : it->second.get();
+}
+
+int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
+  if (prec != ACUM_FP64 && prec != ACUM_MIXED) return ctx_fail(ctx, ACUM_ERR_ARG, "bad precision");
+  if (repeats < 1) repeats = 1;
+  hipEvent_t e0, e1;
+  HIPCHK(ctx, hipEventCreate(&e0));
+  HIPCHK(ctx, hipEventCreate(&e1));
+  HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+  for (auto& sc : st.cases) {
+    int rc = ensure_records(ctx, sc, prec);
+    if (rc) return rc;
+  }
+  HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+  int rc = build_plan(ctx, st, prec);
+  if (rc) return rc;
+  Plan& p = st.plan[prec];
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float prep = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&prep, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  double eval = 0, red = 0, total = 0;
+  if (use_graph) {
+    if (!p.exec) {
+      HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_pass(ctx, st, prec, false);
+      hipGraph_t g = nullptr;
+      hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+      if (rc) return rc;
+      HIPCHK(ctx, ce);
+      p.graph = g;
+      HIPCHK(ctx, hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    for (int r = 0; r < repeats; r++) HIPCHK(ctx, hipGraphLaunch(p.exec, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    total = ms / repeats;
+  } else {
+    HIPCHK(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+    for (int r = 0; r < repeats; r++) {
+      rc = enqueue_pass(ctx, st, prec, true);
+      if (rc) return rc;
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      float a = 0, b = 0;
+      HIPCHK(ctx, hipEventElapsedTime(&a, ctx->ev[1], ctx->ev[2]));
+      HIPCHK(ctx, hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]));
+      eval += a;
+      red += b;
+    }
+    HIPCHK(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+    total = ms / repeats;
+    eval /= repeats;
+    red /= repeats;
+  }
+  acum_timing& t = ctx->timing;
+  t = acum_timing();
+  t.prep_ms = prep;
+  t.eval_ms = eval;
+  t.reduce_ms = red;
+  t.total_ms = total;
+  for (auto& sc : st.cases) t.evals += sc.nE * sc.nT * sc.nObs;
+  t.partial_bytes = p.slab_bytes;
+  t.n_partials = 0;
+  for (int g : p.G) t.n_partials = std::max(t.n_partials, g);
+  t.n_items = (int32_t)p.items.size();
+  t.launches = 3 + (int32_t)st.cases.size();
+  return check_flags(ctx);
+}
+
+acum_tune default_tune(int prec) {
+  acum_tune t{};
+  t.window = 64;
+  t.max_spread = prec == ACUM_MIXED ? 8 : 6;
+  t.max_tile = 256;
+  t.groups = 0;
+  t.max_chunk = 1 << 20;
+  t.waves = 1;
+  return t;
+}
+
+}  // namespace
+
+// =============================================================================================
+extern "C" {
+
+int acum_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int acum_ctx_create(acum_ctx** out, int device) {
+  if (!out) return ACUM_ERR_ARG;
+  *out = nullptr;
+  int n = acum_device_count();
+  if (device < 0 || device >= n) return ACUM_ERR_DEVICE;
+  acum_ctx* c = new acum_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return ACUM_ERR_DEVICE;
+  }
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      delete c;
+      return ACUM_ERR_DEVICE;
+    }
+  if (hipMalloc(&c->d_counter, sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->d_flags, sizeof(uint32_t)) != hipSuccess) {
+    delete c;
+    return ACUM_ERR_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    delete c;
+    return ACUM_ERR_DEVICE;
+  }
+  c->cu_count = prop.multiProcessorCount;
+  c->tune[ACUM_FP64] = default_tune(ACUM_FP64);
+  c->tune[ACUM_MIXED] = default_tune(ACUM_MIXED);
+  *out = c;
+  return ACUM_OK;
+}
+
+void acum_ctx_destroy(acum_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  for (auto& kv : c->stages) free_stage(*kv.second);
+  acum::bpm_release(c);
+  (void)hipFree(c->d_counter);
+  (void)hipFree(c->d_flags);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* acum_last_error(const acum_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int acum_tonal_stage(acum_ctx* ctx, const acum_tonal_case* cases, int32_t nCases, int32_t* handle) {
+  if (!ctx || !cases || nCases < 1 || !handle) return ctx_fail(ctx, ACUM_ERR_ARG, "bad argument");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto st = std::make_unique<Stage>();
+  st->cases.resize(nCases);
+  for (int i = 0; i < nCases; i++) {
+    int rc = stage_case(ctx, &cases[i], st->cases[i]);
+    if (rc) {
+      free_stage(*st);
+      return rc;
+    }
+  }
+  *handle = ctx->next_handle++;
+  ctx->stages[*handle] = std::move(st);
+  return ACUM_OK;
+}
+
+int acum_tonal_run(acum_ctx* ctx, int32_t handle, int32_t precision, int32_t repeats, int32_t use_graph) {
+  if (!ctx) return ACUM_ERR_ARG;
+  Stage* st = get_stage(ctx, handle);
+  if (!st) return ctx_fail(ctx, ACUM_ERR_ARG, "unknown handle");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  return run_stage(ctx, *st, precision, repeats, use_graph != 0);
+}
+
+int acum_tonal_fetch(acum_ctx* ctx, int32_t handle, int32_t icase, float* pT, float* pL, float* pA) {
+  if (!ctx) return ACUM_ERR_ARG;
+  Stage* st = get_stage(ctx, handle);
+  if (!st || icase < 0 || icase >= (int)st->cases.size())
+    return ctx_fail(ctx, ACUM_ERR_ARG, "unknown handle or case");
+  StagedCase& sc = st->cases[icase];
+  const size_t n = (size_t)sc.nT * sc.nObs * sizeof(float);
+  float* dst[3] = {pT, pL, pA};
+  for (int i = 0; i < 3; i++)
+    if (dst[i]) HIPCHK(ctx, hipMemcpy(dst[i], sc.d_out[i], n, hipMemcpyDeviceToHost));
+  return ACUM_OK;
+}
+
+int acum_tonal_release(acum_ctx* ctx, int32_t handle) {
+  if (!ctx) return ACUM_ERR_ARG;
+  auto it = ctx->stages.find(handle);
+  if (it == ctx->stages.end()) return ctx_fail(ctx, ACUM_ERR_ARG, "unknown handle");
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  free_stage(*it->second);
+  ctx->stages.erase(it);
+  return ACUM_OK;
+}
+
+int acum_tonal(acum_ctx* ctx, const acum_tonal_case* c, float* pT, float* pL, float* pA) {
+  if (!ctx || !c || !pT || !pL || !pA) return ctx_fail(ctx, ACUM_ERR_ARG, "bad argument");
+  int32_t h = 0;
+  int rc = acum_tonal_stage(ctx, c, 1, &h);
+  if (rc) return rc;
+  rc = acum_tonal_run(ctx, h, c->precision, 1, 0);
+  if (!rc) rc = acum_tonal_fetch(ctx, h, 0, pT, pL, pA);
+  acum_tonal_release(ctx, h);
+  return rc;
+}
+
+int acum_tonal_batch(acum_ctx* ctx, const acum_tonal_case* cases, int32_t nCases, float* const* pT,
+                     float* const* pL, float* const* pA) {
+  if (!ctx || !cases || nCases < 1 || !pT || !pL || !pA) return ctx_fail(ctx, ACUM_ERR_ARG, "bad argument");
+  for (int i = 1; i < nCases; i++)
+    if (cases[i].precision != cases[0].precision)
+      return ctx_fail(ctx, ACUM_ERR_ARG, "all cases of a batch must use one precision");
+  int32_t h = 0;
+  int rc = acum_tonal_stage(ctx, cases, nCases, &h);
+  if (rc) return rc;
+  rc = acum_tonal_run(ctx, h, cases[0].precision, 1, 0);
+  for (int i = 0; i < nCases && !rc; i++) rc = acum_tonal_fetch(ctx, h, i, pT[i], pL[i], pA[i]);
+  acum_tonal_release(ctx, h);
+  return rc;
+}
+
+int acum_tonal_bins(acum_ctx* ctx, const acum_tonal_case* c, int64_t iobs, int32_t* ipts,
+                    int32_t* npi, int32_t* ndown, int32_t* nup) {
+  if (!ctx || !c || !ipts || !npi || !ndown || !nup) return ctx_fail(ctx, ACUM_ERR_ARG, "bad argument");
+  if (iobs < 0 || iobs >= c->nObs) return ctx_fail(ctx, ACUM_ERR_ARG, "observer out of range");
+  int32_t h = 0;
+  int rc = acum_tonal_stage(ctx, c, 1, &h);
+  if (rc) return rc;
+  Stage& st = *get_stage(ctx, h);
+  StagedCase& sc = st.cases[0];
+  rc = ensure_records(ctx, sc, c->precision);
+  const int64_t n = sc.nE * sc.nT;
+  int32_t* d = nullptr;
+  if (!rc && hipMalloc(&d, 4 * n * sizeof(int32_t)) != hipSuccess) rc = ctx_fail(ctx, ACUM_ERR_NOMEM, "hipMalloc");
+  if (!rc) {
+    const double ox = c->rObs[3 * iobs], oy = c->rObs[3 * iobs + 1], oz = c->rObs[3 * iobs + 2];
+    hipError_t e = launch_tonal_bins(c->precision == ACUM_MIXED, ctx->stream, sc.d_rec[c->precision], sc.nE,
+                                     sc.k, ox, oy, oz, d, d + n, d + 2 * n, d + 3 * n);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    int32_t* dst[4] = {ipts, npi, ndown, nup};
+    for (int i = 0; i < 4 && e == hipSuccess; i++)
+      e = hipMemcpy(dst[i], d + i * n, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = ctx_fail(ctx, ACUM_ERR_DEVICE, hipGetErrorString(e));
+  }
+  (void)hipFree(d);
+  acum_tonal_release(ctx, h);
+  return rc;
+}
+
+int acum_get_timing(const acum_ctx* ctx, acum_timing* t) {
+  if (!ctx || !t) return ACUM_ERR_ARG;
+  *t = ctx->timing;
+  return ACUM_OK;
+}
+
+int acum_get_tuning(const acum_ctx* ctx, int32_t precision, acum_tune* t) {
+  if (!ctx || !t || (precision != ACUM_FP64 && precision != ACUM_MIXED)) return ACUM_ERR_ARG;
+  *t = ctx->tune[precision];
+  return ACUM_OK;
+}
+
+int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t) {
+  if (!ctx || !t || (precision != ACUM_FP64 && precision != ACUM_MIXED)) return ACUM_ERR_ARG;
+  const bool mixed = precision == ACUM_MIXED;
+  if (!(t->window == 32 || t->window == 64 || (mixed && t->window == 128)))
+    return ctx_fail(ctx, ACUM_ERR_ARG, "window must be 32 or 64 (or 128 for the mixed path)");
+  if (t->max_spread < 1 || t->max_tile < 1 || t->max_chunk < 1 || t->groups < 0)
+    return ctx_fail(ctx, ACUM_ERR_ARG, "bad tile limits");
+  ctx->tune[precision] = *t;
+  ctx->tune[precision].waves = 1;
+  return ACUM_OK;
+}
+
+int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* best, double* best_ms) {
+  if (!ctx || (precision != ACUM_FP64 && precision != ACUM_MIXED)) return ACUM_ERR_ARG;
+  Stage* st = get_stage(ctx, handle);
+  if (!st) return ctx_fail(ctx, ACUM_ERR_ARG, "unknown handle");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const bool mixed = precision == ACUM_MIXED;
+  std::vector<acum_tune> cand;
+  const int windows[3] = {32, 64, 128};
+  for (int W : windows) {
+    if (!mixed && W == 128) continue;
+    for (int sp : {W / 16, W / 8, W / 4})
+      for (int mt : {64, 256}) {
+        acum_tune t = default_tune(precision);
+        t.window = W;
+        t.max_spread = std::max(1, sp);
+        t.max_tile = mt;
+        cand.push_back(t);
+      }
+  }
+  const acum_tune keep = ctx->tune[precision];
+  double bms = 1e300;
+  acum_tune bt = keep;
+  for (const acum_tune& t : cand) {
+    ctx->tune[precision] = t;
+    int rc = run_stage(ctx, *st, precision, 1, false);  // warm + plan
+    if (rc == ACUM_OK) rc = run_stage(ctx, *st, precision, 2, false);
+    if (rc == ACUM_OK && ctx->timing.total_ms < bms) {
+      bms = ctx->timing.total_ms;
+      bt = t;
+    }
+  }
+  if (bms == 1e300) {
+    ctx->tune[precision] = keep;
+    return ctx_fail(ctx, ACUM_ERR_PLAN, "no candidate tile shape ran");
+  }
+  ctx->tune[precision] = bt;
+  if (best) *best = bt;
+  if (best_ms) *best_ms = bms;
+  return ACUM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,329 @@
+// ORACLE -- test infrastructure only.  CPU restatement of the reference's tonal path:
+// findPressureTimeHistory (core.cu:18-259) with its helpers acous_dist (core.cu:261-279),
+// sumprod/vecmag (core.cu:281-288), findpts (core.cu:290-322) and bsearch (core.cu:325-355).
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this code.
+//
+// Two instances:
+//   orc_tonal_f64  FP64 everywhere: the parity truth for the GPU paths.
+//   orc_tonal_f32  float integrand and float accumulation with the same FP64 observer time as
+//                  orc_tonal_f64 (the "reference-faithful" CPU baseline).
+// Summation order (the reference's is undefined: unordered float atomics, core.cu:202-226):
+//   per observer -> surface -> element -> source step, each bin accumulated in that order.
+//
+// Observer time and bin lookup are written in the canonical FP64 form shared with the GPU
+// kernels (same operations, same order, no contraction), so bin indices must agree exactly:
+//   a = (|vinf|^2) - a0^2, four_a = 4a, inv_2a = 1/(2a)                  (core.cu:265)
+//   d = obs - src; c = |d|^2; b = -2 (d.vinf); disc = b^2 - four_a*c      (core.cu:266-268)
+//   t = (-b - sqrt(disc)) inv_2a, or (-b + sqrt(disc)) inv_2a if negative  (core.cu:273-276)
+//   r = t a0;  tobs(j) = (j dTau + r inv_a0) oM                            (core.cu:156,190,194)
+//   npi = floor((lpi - tobs) inv_lpi); newx = tobs + npi lpi               (core.cu:298-300)
+//   ipts = largest k in [0,nT) with k dPsi <= newx                        (core.cu:325-355)
+// Deviation, documented: where the reference's bsearch would return -1 (newx < 0, undefined
+// behaviour at core.cu:197), npi is raised by one so that newx lies in [0, lpi].
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+#include "../include/acum.h"
+
+namespace {
+
+struct TimeConst {
+  double a0, inv_a0, dTau, oM, lpi, inv_lpi, dPsi, inv_dPsi, four_a, inv_2a;
+  double vx, vy, vz;
+  int64_t nT;
+  bool still;  // vinf == 0: b vanishes exactly, same bits as the general form
+};
+
+TimeConst make_time_const(const acum_tonal_case* c) {
+  TimeConst k;
+  k.a0 = c->a0;
+  k.inv_a0 = 1.0 / c->a0;
+  k.dTau = c->dTau;
+  k.oM = c->oM;
+  k.nT = c->nTime;
+  k.dPsi = c->dPsi;
+  k.inv_dPsi = 1.0 / c->dPsi;
+  k.lpi = (double)c->nTime * c->dPsi;
+  k.inv_lpi = 1.0 / k.lpi;
+  k.vx = c->vInf[0];
+  k.vy = c->vInf[1];
+  k.vz = c->vInf[2];
+  double vv = k.vx * k.vx;
+  vv = vv + k.vy * k.vy;
+  vv = vv + k.vz * k.vz;
+  double a = vv - k.a0 * k.a0;
+  k.four_a = 4.0 * a;
+  k.inv_2a = 1.0 / (2.0 * a);
+  k.still = (k.vx == 0.0 && k.vy == 0.0 && k.vz == 0.0);
+  return k;
+}
+
+// Convected retarded distance r (acous_dist, core.cu:261-279) in the canonical FP64 form.
+inline double retarded_r(const TimeConst& k, double ox, double oy, double oz,
+                         const float* src) {
+  double dx = ox - (double)src[0];
+  double dy = oy - (double)src[1];
+  double dz = oz - (double)src[2];
+  double c = dx * dx;
+  c = c + dy * dy;
+  c = c + dz * dz;
+  double b = 0.0;
+  if (!k.still) {
+    double bh = dx * k.vx;
+    bh = bh + dy * k.vy;
+    bh = bh + dz * k.vz;
+    b = -2.0 * bh;
+  }
+  double disc = b * b - k.four_a * c;
+  double sq = std::sqrt(disc);
+  double t = (-b - sq) * k.inv_2a;
+  if (t < 0.0) t = (-b + sq) * k.inv_2a;
+  return t * k.a0;
+}
+
+inline double obs_time(const TimeConst& k, int64_t j, double r) {
+  return ((double)j * k.dTau + r * k.inv_a0) * k.oM;
+}
+
+// Bin lookup of one eval (findpts + bsearch, core.cu:169,290-355).  Returns false if the
+// observer time is not finite.
+inline bool find_bin(const TimeConst& k, double ttemp, int64_t* ipts_out, int64_t* npi_out,
+                     double* shift_out) {
+  if (!std::isfinite(ttemp)) return false;
+  double q = std::floor((k.lpi - ttemp) * k.inv_lpi);
+  int64_t npi = (int64_t)q;
+  double shift = (double)npi * k.lpi;
+  double newx = ttemp + shift;
+  if (newx < 0.0) {
+    npi += 1;
+    shift = (double)npi * k.lpi;
+    newx = ttemp + shift;
+  }
+  double f = std::floor(newx * k.inv_dPsi);
+  int64_t ip = (int64_t)f;
+  if (ip < 0) ip = 0;
+  if (ip > k.nT - 1) ip = k.nT - 1;
+  while (ip + 1 < k.nT && (double)(ip + 1) * k.dPsi <= newx) ip++;
+  while (ip > 0 && (double)ip * k.dPsi > newx) ip--;
+  *ipts_out = ip;
+  *npi_out = npi;
+  *shift_out = shift;
+  return true;
+}
+
+struct Eval {
+  double pT, pL;
+};
+
+// Farassat 1A integrand of the impermeable surface (core.cu:68-154), arithmetic type R.
+// rho = rhodot = 1 and U = V (core.cu:71-72,109-111), so Un - Vn = 0 exactly; the dotted normal
+// velocities are forced to zero (core.cu:117-118).  The terms that vanish identically (Mo,
+// Modot, LF2, the (Un-Vn) parts of Ma_n/Man_dot) add an exact zero and are left out.
+template <typename R>
+inline void integrand(const float* X, const float* Nn, const float* Vv, const float* PR,
+                      double ox, double oy, double oz, double r_d, const TimeConst& k,
+                      R rhoRef, R* pThick, R* pLoad) {
+  const R a0 = (R)k.a0;
+  const R r = (R)r_d;
+  const R obs[3] = {(R)ox, (R)oy, (R)oz};
+  const R vinf[3] = {(R)k.vx, (R)k.vy, (R)k.vz};
+  R rvec[3], rhat[3];
+  for (int i = 0; i < 3; i++) rvec[i] = obs[i] - (R)X[i] - vinf[i] * r / a0;  // core.cu:86
+  for (int i = 0; i < 3; i++) rhat[i] = rvec[i] / r;                          // core.cu:87
+  const R V[3] = {(R)Vv[0], (R)Vv[1], (R)Vv[2]};
+  const R Vd[3] = {(R)Vv[3], (R)Vv[4], (R)Vv[5]};
+  const R N[3] = {(R)Nn[0], (R)Nn[1], (R)Nn[2]};
+  const R Nd[3] = {(R)Nn[3], (R)Nn[4], (R)Nn[5]};
+  const R ds = (R)X[3];
+  const R P = (R)PR[0], Pdot = (R)PR[1];
+  auto dot = [](const R* a, const R* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+  const R Mr = dot(V, rhat) / a0;                                    // core.cu:93
+  const R M = std::sqrt(V[0] * V[0] + V[1] * V[1] + V[2] * V[2]) / a0; // core.cu:94
+  const R Mdot_r = dot(Vd, rhat) / a0;                               // core.cu:95
+  const R D1 = (R)1.0 / ((R)4.0 * (R)M_PI * a0);                     // core.cu:53
+  const R om = (R)1.0 - Mr;
+  const R T1 = D1 / (r * (om * om));                                  // core.cu:97-100
+  const R T2 = T1 / om;
+  const R T3 = T1 / r;
+  const R T4 = T3 / om;
+  const R Vn = dot(V, N);                                             // core.cu:102
+  const R Ma_n = rhoRef * Vn;                                         // core.cu:121
+  const R TF2 = Ma_n * Mdot_r * T2 * a0 * ds;                         // core.cu:123
+  const R TN = a0 * Ma_n * (Mr - M * M) * T4 * a0 * ds;               // core.cu:124
+  *pThick = TN + TF2;                                                 // core.cu:128
+  R Li[3], Lid[3];
+  for (int i = 0; i < 3; i++) Li[i] = P * N[i];                       // core.cu:130
+  for (int i = 0; i < 3; i++) Lid[i] = Pdot * N[i] + P * Nd[i];       // core.cu:131
+  const R Lidot_r = dot(Lid, rhat);
+  const R Li_r = dot(Li, rhat);
+  const R Li_m = dot(Li, V) / a0;
+  const R LF1 = Lidot_r * T1 * ds + Li_r * Mdot_r * T2 * ds;          // core.cu:138
+  const R LN = (Li_r - Li_m) * T3 * a0 * ds + Li_r * (Mr - M * M) * T4 * a0 * ds; // core.cu:141
+  *pLoad = LN + LF1;                                                  // core.cu:154
+}
+
+// Runs the tent binning of one eval (core.cu:171-231) and calls add(bin, w) in the
+// reference's order: down-sweep bins first, then up-sweep bins.
+template <typename Add>
+inline bool bin_eval(const TimeConst& k, double ttem, double ttemp, double ttep, int64_t* nd,
+                     int64_t* nu, int64_t* ipts_o, int64_t* npi_o, Add add) {
+  int64_t ipts, npi;
+  double shift;
+  if (!find_bin(k, ttemp, &ipts, &npi, &shift)) return false;
+  const int64_t nT = k.nT;
+  const double ttem_s = ttem + shift;
+  const double dm = ttem - ttemp;
+  int64_t iptm = ipts, cnt = 0;
+  double tObs2 = (double)ipts * k.dPsi;
+  while (tObs2 > ttem_s) {
+    double w = (ttem_s - tObs2) / dm;
+    add(iptm, w);
+    tObs2 -= k.dPsi;
+    iptm -= 1;
+    if (iptm < 0) iptm = nT - 1;
+    if (++cnt > nT) return false;
+  }
+  *nd = cnt;
+  int64_t iptp = (ipts < nT - 1) ? ipts + 1 : 0;
+  tObs2 = (double)iptp * k.dPsi;
+  if (iptp == 0) tObs2 += k.lpi;
+  const double ttep_s = ttep + shift;
+  const double dp = ttep - ttemp;
+  cnt = 0;
+  while (tObs2 < ttep_s) {
+    double w = (ttep_s - tObs2) / dp;
+    add(iptp, w);
+    tObs2 += k.dPsi;
+    iptp += 1;
+    if (iptp > nT - 1) iptp = 0;
+    if (++cnt > nT) return false;
+  }
+  *nu = cnt;
+  *ipts_o = ipts;
+  *npi_o = npi;
+  return true;
+}
+
+template <typename R, typename Out>
+int tonal_impl(const acum_tonal_case* c, Out* pT, Out* pL, Out* pA, int nthreads) {
+  if (!c || c->nTime < 2 || c->nObs < 1 || c->nSurf < 1) return ACUM_ERR_ARG;
+  const TimeConst k = make_time_const(c);
+  const int64_t nT = c->nTime, nObs = c->nObs;
+  int status = ACUM_OK;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+  {
+    std::vector<R> bT(nT), bL(nT), bA(nT);
+    std::vector<double> rr(nT);
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t o = 0; o < nObs; o++) {
+      std::fill(bT.begin(), bT.end(), (R)0);
+      std::fill(bL.begin(), bL.end(), (R)0);
+      std::fill(bA.begin(), bA.end(), (R)0);
+      const double ox = c->rObs[3 * o], oy = c->rObs[3 * o + 1], oz = c->rObs[3 * o + 2];
+      bool ok = true;
+      for (int64_t s = 0; s < c->nSurf && ok; s++) {
+        for (int64_t j = 0; j < c->E[s] && ok; j++) {
+          const float* Xe = c->X + c->XPtr[s] + j * nT * 4;
+          const float* Ne = c->N + c->NPtr[s] + j * nT * 6;
+          const float* Ve = c->V + c->VPtr[s] + j * nT * 6;
+          const float* Pe = c->PR + c->PRPtr[s] + j * nT * 2;
+          // One retarded solve per (element, step); the reference solves three times
+          // (core.cu:84,189,193) with identical operands, hence identical values.
+          for (int64_t t = 0; t < nT; t++) rr[t] = retarded_r(k, ox, oy, oz, Xe + 4 * t);
+          for (int64_t t = 0; t < nT && ok; t++) {
+            const int64_t mm = (t > 0) ? t - 1 : nT - 1;       // core.cu:173-179
+            const int64_t mp = (t < nT - 1) ? t + 1 : 0;       // core.cu:180-186
+            const double ttemp = obs_time(k, t, rr[t]);
+            double ttem = obs_time(k, mm, rr[mm]);
+            if (t == 0) ttem = ttem - k.lpi;
+            double ttep = obs_time(k, mp, rr[mp]);
+            if (t == nT - 1) ttep = ttep + k.lpi;
+            R pth, plo;
+            integrand<R>(Xe + 4 * t, Ne + 6 * t, Ve + 6 * t, Pe + 2 * t, ox, oy, oz, rr[t], k,
+                         (R)c->rhoRef, &pth, &plo);
+            int64_t nd, nu, ip, np;
+            ok = bin_eval(k, ttem, ttemp, ttep, &nd, &nu, &ip, &np, [&](int64_t b, double w) {
+              const R wr = (R)w;
+              bT[b] += pth * wr;
+              bL[b] += plo * wr;
+              bA[b] += (pth + plo) * wr;
+            });
+          }
+        }
+      }
+      if (!ok) {
+#pragma omp critical
+        status = ACUM_ERR_NUMERIC;
+      }
+      for (int64_t t = 0; t < nT; t++) {
+        pT[t * nObs + o] = (Out)bT[t];
+        pL[t * nObs + o] = (Out)bL[t];
+        pA[t * nObs + o] = (Out)bA[t];
+      }
+    }
+  }
+  return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_tonal_f64(const acum_tonal_case* c, double* pT, double* pL, double* pA, int nthreads) {
+  return tonal_impl<double, double>(c, pT, pL, pA, nthreads);
+}
+
+int orc_tonal_f32(const acum_tonal_case* c, float* pT, float* pL, float* pA, int nthreads) {
+  return tonal_impl<float, float>(c, pT, pL, pA, nthreads);
+}
+
+// Bin lookup of every eval of one observer, in case order (surface, element, step).
+int orc_tonal_bins(const acum_tonal_case* c, int64_t iobs, int32_t* ipts, int32_t* npi,
+                   int32_t* ndown, int32_t* nup) {
+  if (!c || iobs < 0 || iobs >= c->nObs) return ACUM_ERR_ARG;
+  const TimeConst k = make_time_const(c);
+  const int64_t nT = c->nTime;
+  const double ox = c->rObs[3 * iobs], oy = c->rObs[3 * iobs + 1], oz = c->rObs[3 * iobs + 2];
+  std::vector<double> rr(nT);
+  int64_t idx = 0;
+  for (int64_t s = 0; s < c->nSurf; s++) {
+    for (int64_t j = 0; j < c->E[s]; j++) {
+      const float* Xe = c->X + c->XPtr[s] + j * nT * 4;
+      for (int64_t t = 0; t < nT; t++) rr[t] = retarded_r(k, ox, oy, oz, Xe + 4 * t);
+      for (int64_t t = 0; t < nT; t++, idx++) {
+        const int64_t mm = (t > 0) ? t - 1 : nT - 1;
+        const int64_t mp = (t < nT - 1) ? t + 1 : 0;
+        const double ttemp = obs_time(k, t, rr[t]);
+        double ttem = obs_time(k, mm, rr[mm]);
+        if (t == 0) ttem = ttem - k.lpi;
+        double ttep = obs_time(k, mp, rr[mp]);
+        if (t == nT - 1) ttep = ttep + k.lpi;
+        int64_t nd = -1, nu = -1, ip = -1, np = 0;
+        if (!bin_eval(k, ttem, ttemp, ttep, &nd, &nu, &ip, &np, [](int64_t, double) {}))
+          return ACUM_ERR_NUMERIC;
+        ipts[idx] = (int32_t)ip;
+        npi[idx] = (int32_t)np;
+        ndown[idx] = (int32_t)nd;
+        nup[idx] = (int32_t)nu;
+      }
+    }
+  }
+  return ACUM_OK;
+}
+
+// Observer time of one (element position, step) in the canonical FP64 form (for tests).
+double orc_obs_time(const acum_tonal_case* c, const float* src, const float* obs, int64_t j) {
+  const TimeConst k = make_time_const(c);
+  double r = retarded_r(k, obs[0], obs[1], obs[2], src);
+  return obs_time(k, j, r);
+}
+
+double orc_retarded_r(const acum_tonal_case* c, const float* src, const float* obs) {
+  const TimeConst k = make_time_const(c);
+  return retarded_r(k, obs[0], obs[1], obs[2], src);
+}
+
+}  // extern "C"

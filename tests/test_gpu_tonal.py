@@ -1,0 +1,131 @@
+"""GPU parity of the tonal Farassat 1A path against the FP64 CPU oracle, through the C-ABI.
+
+Bars (from the path's contract): observer-time bin indices bit-exact; FP64 pressure histories
+within 1e-6 relative L2; the mixed FP32/FP64 path within 0.01 dB OASPL of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import acum3_periodic_amd as acum
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL_L2_FP64 = 1e-6       # FP64 GPU vs FP64 oracle, each of pT, pL, pA
+OASPL_DB = 0.01          # mixed GPU vs FP64 oracle
+REL_L2_MIXED = 2e-3      # mixed GPU vs FP64 oracle on pA (FP32 integrands, FP32 tile partials)
+
+
+def small_case(config=1, precision=acum.FP64, **over):
+    base = dict(nChord=4, nSpan=3, nTime=36, nObs=8)
+    if config == 2:
+        base = dict(nChord=5, nSpan=4, nTime=64, nObsAz=6, nObsEl=3)
+    base.update(over)
+    run = acum.Run.synth(acum.synth_params(config, **base)).preprocess()
+    return run, run.tonal_case(precision)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if acum.device_count() < 1:
+        pytest.fail("no GPU visible")
+    c = acum.Context(0)
+    yield c
+    c.close()
+
+
+CASES = [
+    ("hover", 1, {}),
+    ("hover_odd_obs", 1, dict(nObs=70, nTime=40)),
+    ("flight", 2, {}),
+    ("flight_coarse_time", 2, dict(nTime=24)),
+]
+
+
+@pytest.mark.parametrize("name,config,over", CASES)
+def test_bins_bit_exact(ctx, name, config, over):
+    run, c = small_case(config, acum.FP64, **over)
+    for iobs in (0, c.nObs - 1):
+        g = ctx.bins(c, iobs)
+        o = O.tonal_bins(c, iobs)
+        for a, b, what in zip(g, o, ("ipts", "npi", "ndown", "nup")):
+            assert np.array_equal(a, b), f"{name}: {what} differs at observer {iobs}"
+    # the mixed path uses the same FP64 observer time from the same float positions
+    cm = run.tonal_case(acum.MIXED)
+    g = ctx.bins(cm, 0)
+    o = O.tonal_bins(cm, 0)
+    assert all(np.array_equal(a, b) for a, b in zip(g, o))
+
+
+@pytest.mark.parametrize("name,config,over", CASES)
+def test_fp64_matches_oracle(ctx, name, config, over):
+    run, c = small_case(config, acum.FP64, **over)
+    gT, gL, gA = ctx.tonal(c)
+    oT, oL, oA = O.tonal_f64(c)
+    for g, o, what in ((gT, oT, "pT"), (gL, oL, "pL"), (gA, oA, "pA")):
+        err = O.rel_l2(g, o)
+        assert err <= REL_L2_FP64, f"{name} {what}: rel L2 {err:.3e}"
+
+
+@pytest.mark.parametrize("name,config,over", CASES)
+def test_mixed_matches_oracle(ctx, name, config, over):
+    run, c = small_case(config, acum.MIXED, **over)
+    gT, gL, gA = ctx.tonal(c)
+    oT, oL, oA = O.tonal_f64(c)
+    err = O.rel_l2(gA, oA)
+    assert err <= REL_L2_MIXED, f"{name}: pA rel L2 {err:.3e}"
+    d = np.abs(acum.oaspl(gA) - acum.oaspl(oA.astype(np.float32)))
+    assert d.max() <= OASPL_DB, f"{name}: OASPL differs by {d.max():.4f} dB"
+
+
+def test_deterministic_and_graph(ctx):
+    run, c = small_case(2, acum.MIXED)
+    h = ctx.stage([c])
+    ctx.run(h, acum.MIXED)
+    a = ctx.fetch(h, 0, c.nTime, c.nObs)
+    ctx.run(h, acum.MIXED)
+    b = ctx.fetch(h, 0, c.nTime, c.nObs)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    for rep in range(3):  # replays of one captured graph, with eager work in between
+        ctx.run(h, acum.MIXED, repeats=2, graph=True)
+        g = ctx.fetch(h, 0, c.nTime, c.nObs)
+        for x, y in zip(a, g):
+            assert np.array_equal(x, y), f"graph replay {rep} differs"
+        ctx.run(h, acum.FP64)
+    ctx.release(h)
+
+
+def test_batch_equals_single(ctx):
+    runs = [small_case(1, acum.MIXED, nTime=32 + 4 * i, nObs=5 + 30 * i, nSpan=2 + i) for i in range(3)]
+    cases = [c for _, c in runs]
+    batch = ctx.tonal_batch(cases)
+    for c, b in zip(cases, batch):
+        s = ctx.tonal(c)
+        for x, y in zip(s, b):
+            assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("window,spread", [(32, 2), (64, 16), (128, 16)])
+def test_tile_shapes_agree(ctx, window, spread):
+    run, c = small_case(2, acum.MIXED)
+    ref = ctx.tonal(c)[2]
+    ctx.set_tuning(acum.MIXED, window=window, max_spread=spread, max_tile=32)
+    try:
+        got = ctx.tonal(c)[2]
+    finally:
+        ctx.set_tuning(acum.MIXED, window=64, max_spread=8, max_tile=256)
+    assert O.rel_l2(got, ref) < 1e-5
+
+
+def test_errors_are_returned(ctx):
+    run, c = small_case(1, acum.FP64)
+    c.vInf[0] = 400.0  # supersonic free stream
+    with pytest.raises(acum.AcumError) as e:
+        ctx.tonal(c)
+    assert e.value.code == acum.ACUM_ERR_NUMERIC
+    run, c = small_case(1, acum.FP64)
+    c.precision = 7
+    with pytest.raises(acum.AcumError) as e:
+        ctx.tonal(c)
+    assert e.value.code == acum.ACUM_ERR_ARG
