@@ -421,10 +421,12 @@ int acum_synth(const acum_synth_params* p, acum_run** out) {
           double bl[4];
           bl_over_chord(4.0, re, bl);
           for (int w = 0; w < 4; w++) bd[w] = (float)(bl[w] * p->chord);
+          // section coefficients 0.5 + 0.05 sin psi and 0.01 (plus noise), written normalised by
+          // the tip speed as the file contract asks: the reader divides them by (r/R)^2
           const double cn = 0.5 + 0.05 * std::sin(psi) + 0.002 * g.normal();
           const double cc = 0.01 + 0.002 * g.normal();
-          bd[4] = (float)cn;  // file values; the reader divides by (r/R)^2 below
-          bd[5] = (float)cc;
+          bd[4] = (float)(cn * rR * rR);
+          bd[5] = (float)(cc * rR * rR);
           bd[6] = (float)(p->twistRoot + (p->twistTip - p->twistRoot) * (rR - 0.2) / 0.8);
         }
       }

@@ -578,3 +578,66 @@ int orc_bpm_f64(const acum_bpm_case* c, int64_t obs0, int64_t nobs, double* BBsp
 }
 
 }  // extern "C"
+
+extern "C" {
+// One call of core() (bb_core.cu:591) adding into out[nf2*7] (dB), for unit tests.
+void orc_bpm_core(double rpnorm, double theta, double phi, double c, double lsect, double alpha,
+                  double u, double a0, int trip, double del_s, double del_p, double dis_s,
+                  double dis_p, int isect, int nsect, const float* ff, int nf2, double R,
+                  double* out) {
+  core(rpnorm, theta, phi, c, lsect, alpha, u, 0.0000181206 / 1.22500, a0, trip, del_s, del_p,
+       dis_s, dis_p, isect, nsect, ff, nf2, R, out);
+}
+}
+
+extern "C" {
+// Arguments of every core() call for observer `o`, in the reference's order (surface, section,
+// step): rpnorm, theta, phi, c, lsect, alpha, u, del_s, del_p, dis_s, dis_p, isect, nsect, span.
+int orc_bpm_core_args(const acum_bpm_case* c, int64_t o, double* out) {
+  const int nS = c->nSurf_BB, nT = c->nTime;
+  int64_t k = 0;
+  for (int s = 0; s < nS; s++) {
+    std::vector<SectState> st;
+    if (!prepare_surface(c, s, st)) return ACUM_ERR_ARG;
+    const int ns = c->nsect[s];
+    const float* xyzc = c->xyzcR + c->xsPtr[s];
+    const double span = c->span[s];
+    for (int is = 0; is < ns; is++) {
+      double lsect;
+      if (is == 0) lsect = ((double)xyzc[4 * (is + 1)] - xyzc[4 * is]) / 2.0 * span;
+      else if (is == ns - 1) lsect = ((double)xyzc[4 * is] - xyzc[4 * (is - 1)]) / 2.0 * span;
+      else lsect = ((double)xyzc[4 * (is + 1)] - xyzc[4 * (is - 1)]) / 2.0 * span;
+      const int km = (is == ns - 1) ? is - 1 : is, kp = (is == ns - 1) ? is : is + 1;
+      for (int it = 0; it < nT; it++) {
+        const SectState& q = st[(size_t)is * nT + it];
+        const SectState& qm = st[(size_t)km * nT + it];
+        const SectState& qp = st[(size_t)kp * nT + it];
+        double tr[3][3], tmp = 0;
+        for (int i = 0; i < 3; i++) { tr[0][i] = q.xyzt[i] - q.xyz[i]; tmp += p2(tr[0][i]); }
+        for (int i = 0; i < 3; i++) tr[0][i] /= std::sqrt(tmp);
+        tmp = 0;
+        for (int i = 0; i < 3; i++) { tr[1][i] = qp.xyz[i] - qm.xyz[i]; tmp += p2(tr[1][i]); }
+        for (int i = 0; i < 3; i++) tr[1][i] /= std::sqrt(tmp);
+        tr[2][0] = tr[0][1] * tr[1][2] - tr[0][2] * tr[1][1];
+        tr[2][1] = tr[0][2] * tr[1][0] - tr[0][0] * tr[1][2];
+        tr[2][2] = tr[0][0] * tr[1][1] - tr[0][1] * tr[1][0];
+        tmp = std::sqrt(p2(tr[2][0]) + p2(tr[2][1]) + p2(tr[2][2]));
+        for (int i = 0; i < 3; i++) tr[2][i] /= tmp;
+        double rp[3];
+        for (int i = 0; i < 3; i++) rp[i] = c->rObs[3 * o + i] - q.xyzt[i];
+        const double rpn = std::sqrt(p2(rp[0]) + p2(rp[1]) + p2(rp[2]));
+        double rp2[3];
+        for (int i = 0; i < 3; i++) rp2[i] = rp[0] * tr[i][0] + rp[1] * tr[i][1] + rp[2] * tr[i][2];
+        const double r_e = std::sqrt(p2(rp2[1]) + p2(rp2[2]));
+        double* a = out + k * 14;
+        a[0] = rpn; a[1] = std::acos(std::fabs(rp2[0] / rpn)); a[2] = std::acos(std::fabs(rp2[1]) / r_e);
+        a[3] = xyzc[4 * is + 3] * span; a[4] = lsect; a[5] = q.alpha; a[6] = q.u;
+        for (int i = 0; i < 4; i++) a[7 + i] = q.bl[i];
+        a[11] = is; a[12] = ns; a[13] = span;
+        k++;
+      }
+    }
+  }
+  return ACUM_OK;
+}
+}

@@ -112,3 +112,66 @@ def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ---- the reference's own host code (oracle/_ref), for pinning the oracle --------------------
+_ref = None
+
+
+def ref_lib():
+    """libref_host.so, built by oracle/Makefile from the reference sources (None if absent)."""
+    global _ref
+    if _ref is None and ref_available():
+        _ref = C.CDLL(REF_PATH)
+    return _ref
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def ref_bpm(case, obs0=0, nobs=0):
+    """findBBnoise (bb_core.cu:11) of the reference build on a broadband case (float)."""
+    lib = ref_lib()
+    nS, nT, nf2 = case.nSurf_BB, case.nTime, case.nf2
+    nobs = nobs if nobs > 0 else case.nObs
+    get = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy()
+    nsect = get(case.nsect, nS)
+    tot = int(nsect.sum())
+    nst = get(case.nsect_t, nS)
+    nm = get(case.nmach_t, nS)
+    na = get(case.naoa_t, nS)
+    nr = get(case.nre_t, nS)
+    arrs = dict(
+        span=_f32(get(case.span, nS)), omega=_f32(get(case.omega, nS)), ccw=_i32(get(case.ccw, nS)),
+        psi=_f32(get(case.psi_offset, nS)), trip=_i32(get(case.trip, nS)),
+        rot=_f32(get(case.rotate, 3 * nS)), tra=_f32(get(case.translate, 3 * nS)),
+        xs=_i32(get(case.xsPtr, nS)), ds=_i32(get(case.dsPtr, nS)), nsect=_i32(nsect),
+        xyzc=_f32(get(case.xyzcR, 4 * tot)), bb=_f32(get(case.bbdata, 7 * tot * nT)),
+        nst=_i32(nst), nm=_i32(nm), na=_i32(na), nr=_i32(nr),
+        aP=_i32(get(case.aPtr, nS)), mP=_i32(get(case.mPtr, nS)), rP=_i32(get(case.rPtr, nS)),
+        sP=_i32(get(case.sPtr, nS)), dP=_i32(get(case.dPtr, nS)),
+        st=_f32(get(case.sect_t, int(nst.sum()))), mt=_f32(get(case.mach_t, int(nm.sum()))),
+        at=_f32(get(case.aoa_t, int(na.sum()))), rt=_f32(get(case.re_t, int(nr.sum()))),
+        bt=_f32(get(case.bbdata_t, int((nst * nm * na * nr * 7).sum()))),
+        vinf=_f32([case.vInf[0], case.vInf[1], case.vInf[2]]),
+        robs=_f32(get(case.rObs, 3 * case.nObs)[3 * obs0:3 * (obs0 + nobs)]),
+        tobs=_f32(np.zeros(nT)), ff=_f32(get(case.ff, nf2)))
+    spl = np.full(nobs * nf2 * 7, -100.0, np.float32)
+    oa = np.full(nobs, -100.0, np.float32)
+    fn = lib._Z11findBBnoiseiiifPfS_PiS_S0_S_S_S0_S0_S0_S_S_S0_S0_S0_S0_S0_S0_S0_S0_S0_S_S_S_S_S_iS_S_S_iS_fS_S_
+    fn.restype = None
+    ip = lambda a: a.ctypes.data_as(_i32p)
+    fp = lambda a: a.ctypes.data_as(_f32p)
+    a = arrs
+    fn(C.c_int(nS), C.c_int(nobs), C.c_int(nT), C.c_float(case.dTau), fp(a["span"]), fp(a["omega"]),
+       ip(a["ccw"]), fp(a["psi"]), ip(a["trip"]), fp(a["rot"]), fp(a["tra"]), ip(a["xs"]), ip(a["ds"]),
+       ip(a["nsect"]), fp(a["xyzc"]), fp(a["bb"]), ip(a["nst"]), ip(a["nm"]), ip(a["na"]), ip(a["nr"]),
+       ip(a["aP"]), ip(a["mP"]), ip(a["rP"]), ip(a["sP"]), ip(a["dP"]), fp(a["st"]), fp(a["mt"]),
+       fp(a["at"]), fp(a["rt"]), fp(a["bt"]), C.c_int(case.t_del), fp(a["vinf"]), fp(a["robs"]),
+       fp(a["tobs"]), C.c_int(nf2), fp(a["ff"]), C.c_float(case.a0), fp(spl), fp(oa))
+    return spl.reshape(nobs, nf2, 7), oa
