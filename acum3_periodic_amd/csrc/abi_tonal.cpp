@@ -132,13 +132,8 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   TimeConst k{};
   k.a0 = c->a0;
   k.inv_a0 = 1.0 / c->a0;
-  k.dTau = c->dTau;
-  k.oM = c->oM;
+  k.r2b = (c->oM / c->a0) / c->dPsi;
   k.nT = (int32_t)c->nTime;
-  k.dPsi = c->dPsi;
-  k.inv_dPsi = 1.0 / c->dPsi;
-  k.lpi = (double)c->nTime * c->dPsi;
-  k.inv_lpi = 1.0 / k.lpi;
   k.vx = c->vInf[0];
   k.vy = c->vInf[1];
   k.vz = c->vInf[2];
@@ -227,7 +222,7 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
   if (sc.d_rec[prec]) return ACUM_OK;
   if (!sc.d_aos[0]) return ctx_fail(ctx, ACUM_ERR_ARG, "staged element data already released");
   const bool mixed = prec == ACUM_MIXED;
-  const size_t bytes = (size_t)sc.nE * sc.nT * kRecFields * (mixed ? sizeof(float) : sizeof(double));
+  const size_t bytes = (size_t)sc.nE * sc.nT * (mixed ? sizeof(Rec32) : sizeof(Rec64));
   HIPCHK(ctx, hipMalloc(&sc.d_rec[prec], bytes));
   HIPCHK(ctx, launch_tonal_prep(mixed, ctx->stream, sc.d_aos[0], sc.d_aos[1], sc.d_aos[2], sc.d_aos[3],
                                 sc.nE * sc.nT, sc.k.inv_a0, sc.d_rec[prec]));
@@ -244,10 +239,11 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
 // Tiles of one case: runs of consecutive elements of one surface whose positions stay within
 // `max_spread` observer-time bins of the first element's at every step (Lipschitz bound of the
 // convected retarded distance: |dr| <= |dx| / (1 - Minf)), with the pass length K chosen so the
-// bins of a pass fit the window:  (K + 1) * step_bins + 2 * spread + 6 <= W.
+// bins of a pass, with a 3-bin margin each side for the 4-slot footprint, fit the window:
+// (K + 1) * step_bins + 2 * spread + 8 <= W.
 int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vector<Tile>& tiles) {
   const int64_t nT = sc.nT;
-  const double L = sc.k.oM / (sc.k.a0 * (1.0 - sc.Minf) * sc.k.dPsi);  // bins per metre
+  const double L = sc.k.r2b / (1.0 - sc.Minf);  // bins per metre
   const int W = t.window;
   const float* P = sc.pos.data();
   auto dist2 = [&](int64_t e1, int64_t k1, int64_t e2, int64_t k2) {
@@ -265,7 +261,7 @@ int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vec
   }
   auto pass_len = [&](int spread, double S) {
     const double sb = 1.0 + S * L;
-    const int K = (int)std::floor((W - 2.0 * spread - 6.0) / sb) - 1;
+    const int K = (int)std::floor((W - 2.0 * spread - 8.0) / sb) - 1;
     return std::min<int64_t>(std::min<int64_t>(K, t.max_chunk), nT);
   };
   const size_t nsurf = sc.surf0.size();
@@ -707,8 +703,8 @@ int acum_get_tuning(const acum_ctx* ctx, int32_t precision, acum_tune* t) {
 int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t) {
   if (!ctx || !t || (precision != ACUM_FP64 && precision != ACUM_MIXED)) return ACUM_ERR_ARG;
   const bool mixed = precision == ACUM_MIXED;
-  if (!(t->window == 32 || t->window == 64 || (mixed && t->window == 128)))
-    return ctx_fail(ctx, ACUM_ERR_ARG, "window must be 32 or 64 (or 128 for the mixed path)");
+  if (!(t->window == 32 || t->window == 64 || (mixed && (t->window == 48 || t->window == 128))))
+    return ctx_fail(ctx, ACUM_ERR_ARG, "window must be 32 or 64 (or 48, 128 for the mixed path)");
   if (t->max_spread < 1 || t->max_tile < 1 || t->max_chunk < 1 || t->groups < 0)
     return ctx_fail(ctx, ACUM_ERR_ARG, "bad tile limits");
   ctx->tune[precision] = *t;
@@ -723,9 +719,9 @@ int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* b
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const bool mixed = precision == ACUM_MIXED;
   std::vector<acum_tune> cand;
-  const int windows[3] = {32, 64, 128};
+  const int windows[4] = {32, 48, 64, 128};
   for (int W : windows) {
-    if (!mixed && W == 128) continue;
+    if (!mixed && (W == 48 || W == 128)) continue;
     for (int sp : {W / 16, W / 8, W / 4})
       for (int mt : {64, 256}) {
         acum_tune t = default_tune(precision);

@@ -5,15 +5,32 @@
 namespace acum {
 
 // Element-time record: one (element, source step), wave-uniform in the tonal kernel.
-// Fields: 0-2 position, 3 dS, 4-6 V, 7-9 dV/dt, 10-12 n, 13-15 dn/dt, 16 P, 17 dP/dt,
-// 18 M^2 = (|V|/a0)^2, 19 V.n   (the last two are observer independent, core.cu:94,102).
-constexpr int kRecFields = 20;
+// The 18 input values stay float (they are parsed floats); the two observer-independent
+// products M^2 = (|V|/a0)^2 (core.cu:94) and V.n (core.cu:102) are formed in FP64 and kept
+// in the path's precision.
+struct Rec32 {
+  float x[3], ds, v[3], vd[3], n[3], nd[3], P, dP;
+  float MM, Vn;
+};  // 80 bytes
+struct Rec64 {
+  float x[3], ds, v[3], vd[3], n[3], nd[3], P, dP;
+  double MM, Vn;
+};  // 88 bytes
 
-// Canonical FP64 observer-time constants; the CPU oracle forms the same values.
+// Canonical FP64 observer time, in units of the observer-time bin dPsi; the CPU oracle forms
+// the same values with the same operations.
+//   c = fma(dz,dz, fma(dy,dy, dx*dx))                         |obs - src|^2
+//   still air: r = sqrt(c)
+//   otherwise: b = -2 fma(dz,vz, fma(dy,vy, dx*vx)); disc = fma(b, b, -(four_a c));
+//              t = (-b - sqrt(disc)) inv_2a, or (-b + sqrt(disc)) inv_2a if negative; r = t a0
+//   tau(j) = fma(r, r2b, j)       r2b = (oM / a0) / dPsi: (j dTau + r / a0) oM / dPsi
+// The bins of an eval are the integers b in (tau_prev, tau] (rising side of the tent) and in
+// (tau, tau_next) (falling side); bin b is observer time b dPsi, stored at b mod nT.
 struct TimeConst {
-  double a0, inv_a0, dTau, oM, lpi, inv_lpi, dPsi, inv_dPsi, four_a, inv_2a;
+  double a0, r2b, four_a, inv_2a;
   double vx, vy, vz;
-  int32_t nT, still;  // still: vinf == 0, the b term vanishes exactly
+  double inv_a0;
+  int32_t nT, still;  // still: vinf == 0
 };
 
 struct TonalCaseDev {

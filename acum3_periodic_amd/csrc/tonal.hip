@@ -27,116 +27,105 @@
 namespace acum {
 
 // ---------------------------------------------------------------------------------------------
-// Canonical FP64 observer time (acous_dist core.cu:261-279, ttemp core.cu:156).
+// Canonical FP64 observer time (acous_dist core.cu:261-279, ttemp core.cu:156) in bin units; the
+// CPU oracle evaluates the same operations in the same order.
 __device__ __forceinline__ double retarded_r(const TimeConst& k, double dx, double dy, double dz) {
-  double c = dx * dx;
-  c = c + dy * dy;
-  c = c + dz * dz;
-  double b = 0.0;
-  if (!k.still) {
-    double bh = dx * k.vx;
-    bh = bh + dy * k.vy;
-    bh = bh + dz * k.vz;
-    b = -2.0 * bh;
-  }
-  const double disc = b * b - k.four_a * c;
+  const double c = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+  if (k.still) return sqrt(c);
+  const double b = -2.0 * __builtin_fma(dz, k.vz, __builtin_fma(dy, k.vy, dx * k.vx));
+  const double disc = __builtin_fma(b, b, -(k.four_a * c));
   const double sq = sqrt(disc);
   double t = (-b - sq) * k.inv_2a;
   if (t < 0.0) t = (-b + sq) * k.inv_2a;
   return t * k.a0;
 }
 
-__device__ __forceinline__ double obs_time(const TimeConst& k, int j, double r) {
-  return ((double)j * k.dTau + r * k.inv_a0) * k.oM;
+__device__ __forceinline__ double obs_tau(const TimeConst& k, int j, double r) {
+  return __builtin_fma(r, k.r2b, (double)j);
 }
 
-// Tent binning of one eval (findpts/bsearch core.cu:290-355, sweeps core.cu:196-231).
-// Calls add(absolute_bin, t_num, t_den_inverse_index) for every bin: absolute bin a = shifted
-// index - npi*nT, so a*dPsi is the unshifted observer time of the bin.  Returns the lookup
-// (ipts, npi) and sweep lengths for diagnostics.  Weight of a bin is num/den with
-//   down-sweep: num = ttem_s - tObs2, den = ttem - ttemp
-//   up-sweep:   num = ttep_s - tObs2, den = ttep - ttemp
-template <typename AddDown, typename AddUp>
-__device__ __forceinline__ bool bin_sweeps(const TimeConst& k, double ttem, double ttemp,
-                                           double ttep, int maxbins, int* ipts_o, int* npi_o,
-                                           int* nd_o, int* nu_o, AddDown addd, AddUp addu) {
-  const double q = floor((k.lpi - ttemp) * k.inv_lpi);
-  if (!(fabs(q) < 1.0e9)) return false;  // non-finite observer time
-  int npi = (int)q;
-  double shift = (double)npi * k.lpi;
-  double newx = ttemp + shift;
-  if (newx < 0.0) {
-    npi += 1;
-    shift = (double)npi * k.lpi;
-    newx = ttemp + shift;
-  }
-  const int nT = k.nT;
-  double f = floor(newx * k.inv_dPsi);
-  int ip = (f < 0.0) ? 0 : (f > (double)(nT - 1) ? nT - 1 : (int)f);
-  while (ip + 1 < nT && (double)(ip + 1) * k.dPsi <= newx) ip++;
-  while (ip > 0 && (double)ip * k.dPsi > newx) ip--;
-  const int aoff = -npi * nT;
-  // down-sweep
-  const double ttem_s = ttem + shift;
-  double tObs2 = (double)ip * k.dPsi;
-  int sidx = ip, cnt = 0;
-  bool ok = true;
-  while (tObs2 > ttem_s) {
-    addd(sidx + aoff, ttem_s - tObs2);
-    tObs2 -= k.dPsi;
-    sidx -= 1;
-    if (++cnt > maxbins) {
-      ok = false;
-      break;
-    }
-  }
-  *nd_o = cnt;
-  // up-sweep
-  const int iptp = (ip < nT - 1) ? ip + 1 : 0;
-  tObs2 = (double)iptp * k.dPsi;
-  if (iptp == 0) tObs2 += k.lpi;
-  sidx = ip + 1;
-  const double ttep_s = ttep + shift;
-  cnt = 0;
-  while (ok && tObs2 < ttep_s) {
-    addu(sidx + aoff, ttep_s - tObs2);
-    tObs2 += k.dPsi;
-    sidx += 1;
-    if (++cnt > maxbins) {
-      ok = false;
-      break;
-    }
-  }
-  *nu_o = cnt;
-  *ipts_o = ip;
-  *npi_o = npi;
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
+
+// Tent of one eval (findpts/bsearch core.cu:290-355 and the sweeps core.cu:196-231, in bin
+// units).  The top rising bin is ic = floor(tau); the rising side holds nd = ic - floor(tau_prev)
+// bins ic, ic-1, ... with weights (b - tau_prev) / (tau - tau_prev); the falling side holds
+// nu = ceil(tau_next) - 1 - ic bins ic+1, ic+2, ... with weights (tau_next - b) / (tau_next - tau).
+struct Tent {
+  double fc, tp, tn;  // floor(tau), tau_prev, tau_next
+  int ic, nd, nu;
+};
+
+__device__ __forceinline__ bool tent_of(double tp, double tc, double tn, Tent* T) {
+  const double fc = floor(tc), fp = floor(tp), cn = ceil(tn);
+  const bool ok = fabs(tc) < 1.0e9 && fabs(tp) < 1.0e9 && fabs(tn) < 1.0e9;  // finite, in range
+  T->fc = fc;
+  T->tp = tp;
+  T->tn = tn;
+  T->ic = ok ? (int)fc : 0;
+  const int nd = ok ? (int)(fc - fp) : 0, nu = ok ? (int)(cn - fc) - 1 : 0;
+  T->nd = nd > 0 ? nd : 0;
+  T->nu = nu > 0 ? nu : 0;
   return ok;
 }
 
 // ---------------------------------------------------------------------------------------------
-// Arithmetic helpers per precision.
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcpT(double x) { return 1.0 / x; }
+
+// Records are read through global-address-space pointers: with a wave-uniform address each
+// read is one vector memory request serving all 64 lanes, counted by vmcnt, so waiting for a
+// record never waits for the LDS atomics of the bins (which count on lgkmcnt).
+template <typename T>
+using cptr = const __attribute__((address_space(1))) T*;
+
+// Loads one record as whole dwords.
+template <typename Rec>
+__device__ __forceinline__ Rec load_rec(cptr<Rec> p) {
+  static_assert(sizeof(Rec) % 4 == 0, "record must be whole dwords");
+  constexpr int n = (int)(sizeof(Rec) / 4);
+  cptr<uint32_t> w = (cptr<uint32_t>)p;
+  uint32_t t[n];
+#pragma unroll
+  for (int i = 0; i < n; i++) t[i] = w[i];
+  Rec r;
+  __builtin_memcpy(&r, t, sizeof(Rec));
+  return r;
+}
+
+template <bool MIXED>
+struct Prec {
+  using R = double;
+  using R2 = double2;
+  using Rec = Rec64;
+};
+template <>
+struct Prec<true> {
+  using R = float;
+  using R2 = float2;
+  using Rec = Rec32;
+};
 
 // Farassat 1A integrand of the impermeable surface (core.cu:68-154): rho = 1, U = V, the dotted
 // normal velocities forced to zero (core.cu:117-118), so
 //   pThick = rhoRef Vn dS a0 (dMr T2 + a0 (Mr - M^2) T4)
 //   pLoad  = dS (dLr T1 + Lr dMr T2 + a0 (Lr - Lm) T3 + a0 Lr (Mr - M^2) T4)
 // with T1 = 1/(4 pi a0 r (1-Mr)^2), T2 = T1/(1-Mr), T3 = T1/r, T4 = T3/(1-Mr).
-template <typename R>
-__device__ __forceinline__ void integrand(const R* __restrict__ q, R dx, R dy, R dz, R r,
-                                          R inv_a0, R a0, R vx, R vy, R vz, R rhoRef, R D1,
-                                          R* pth, R* plo, bool* bad) {
+template <typename R, typename Rec>
+__device__ __forceinline__ void integrand(const Rec& q, R ox, R oy, R oz, R r, R inv_a0, R a0,
+                                          R vx, R vy, R vz, R rhoRef, R D1, R* pth, R* plo,
+                                          bool* bad) {
   const R rt = r * inv_a0;
-  const R rvx = fmaT(-vx, rt, dx), rvy = fmaT(-vy, rt, dy), rvz = fmaT(-vz, rt, dz);
+  const R rvx = fmaT(-vx, rt, ox - (R)q.x[0]);
+  const R rvy = fmaT(-vy, rt, oy - (R)q.x[1]);
+  const R rvz = fmaT(-vz, rt, oz - (R)q.x[2]);
   const R ir = rcpT(r);
   const R hx = rvx * ir, hy = rvy * ir, hz = rvz * ir;
-  const R Vr = fmaT(q[4], hx, fmaT(q[5], hy, q[6] * hz));
-  const R dVr = fmaT(q[7], hx, fmaT(q[8], hy, q[9] * hz));
-  const R Nr = fmaT(q[10], hx, fmaT(q[11], hy, q[12] * hz));
-  const R dNr = fmaT(q[13], hx, fmaT(q[14], hy, q[15] * hz));
+  const R Vr = fmaT((R)q.v[0], hx, fmaT((R)q.v[1], hy, (R)q.v[2] * hz));
+  const R dVr = fmaT((R)q.vd[0], hx, fmaT((R)q.vd[1], hy, (R)q.vd[2] * hz));
+  const R Nr = fmaT((R)q.n[0], hx, fmaT((R)q.n[1], hy, (R)q.n[2] * hz));
+  const R dNr = fmaT((R)q.nd[0], hx, fmaT((R)q.nd[1], hy, (R)q.nd[2] * hz));
   const R Mr = Vr * inv_a0;
   const R dMr = dVr * inv_a0;
   const R om = (R)1 - Mr;
@@ -145,8 +134,8 @@ __device__ __forceinline__ void integrand(const R* __restrict__ q, R dx, R dy, R
   const R T2 = T1 * iom;
   const R T3 = T1 * ir;
   const R T4 = T3 * iom;
-  const R MmM = Mr - q[18];
-  const R ds = q[3], P = q[16], dP = q[17], Vn = q[19];
+  const R MmM = Mr - (R)q.MM;
+  const R ds = (R)q.ds, P = (R)q.P, dP = (R)q.dP, Vn = (R)q.Vn;
   *pth = rhoRef * Vn * ds * a0 * fmaT(dMr, T2, a0 * MmM * T4);
   const R Lr = P * Nr;
   const R dLr = fmaT(dP, Nr, P * dNr);
@@ -159,14 +148,13 @@ __device__ __forceinline__ void integrand(const R* __restrict__ q, R dx, R dy, R
   *bad = !(r > (R)0) || !(om > (R)0);
 }
 
-template <bool MIXED>
-struct Prec {
-  using R = double;
-};
-template <>
-struct Prec<true> {
-  using R = float;
-};
+template <typename Rec>
+__device__ __forceinline__ double tau_at(const TimeConst& k, const Rec& q, int j, double ox,
+                                         double oy, double oz, double* r_out) {
+  const double r = retarded_r(k, ox - (double)q.x[0], oy - (double)q.x[1], oz - (double)q.x[2]);
+  *r_out = r;
+  return obs_tau(k, j, r);
+}
 
 // ---------------------------------------------------------------------------------------------
 // Eval + binning.  One wave per workgroup, persistent over a queue of items.
@@ -177,31 +165,33 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
                                                         int* __restrict__ counter,
                                                         uint32_t* __restrict__ flags_out) {
   using R = typename Prec<MIXED>::R;
-  __shared__ R winT[W * 64];
-  __shared__ R winL[W * 64];
+  using Rec = typename Prec<MIXED>::Rec;
+  using R2 = typename Prec<MIXED>::R2;
+  // win[slot][lane] = (thickness, loading) partial of one observer-time bin; each lane owns its
+  // column, so plain read-modify-writes are race free and keep a fixed order
+  __shared__ R2 win[W * 64];
   const int lane = threadIdx.x;
-  for (int s = 0; s < W; s++) {
-    winT[s * 64 + lane] = (R)0;
-    winL[s * 64 + lane] = (R)0;
-  }
+  for (int s = 0; s < W; s++) win[s * 64 + lane] = R2{(R)0, (R)0};
   uint32_t flags = 0;
   for (;;) {
     int it = 0;
     if (lane == 0) it = atomicAdd(counter, 1);
-    it = __shfl(it, 0);
+    it = __builtin_amdgcn_readfirstlane(it);  // lane 0's value, wave-uniform
     if (it >= nItems) break;
     const Item item = items[it];
-    const TonalCaseDev& C = cases[item.icase];
-    const TimeConst k = C.k;
-    const int nT = k.nT;
+    const TonalCaseDev* Cp = cases + item.icase;
+    const TimeConst k = Cp->k;
+    const int nT = k.nT, nObs = Cp->nObs;
     const int o = item.obs0 + lane;
-    const bool active = o < C.nObs;
-    const int oc = active ? o : C.nObs - 1;
-    const double ox = C.obs[3 * oc], oy = C.obs[3 * oc + 1], oz = C.obs[3 * oc + 2];
-    R* slab = (R*)C.slab + ((int64_t)item.group * C.nObs + oc) * (int64_t)nT * 2;
-    const R* rec = (const R*)C.rec;
+    const bool active = o < nObs;
+    const int oc = active ? o : nObs - 1;
+    const double ox = Cp->obs[3 * oc], oy = Cp->obs[3 * oc + 1], oz = Cp->obs[3 * oc + 2];
+    const R oxr = (R)ox, oyr = (R)oy, ozr = (R)oz;
+    R* slab = (R*)Cp->slab + ((int64_t)item.group * nObs + oc) * (int64_t)nT * 2;
+    cptr<Rec> rec = (cptr<Rec>)(uintptr_t)Cp->rec;
     const R inv_a0 = (R)k.inv_a0, a0 = (R)k.a0, vx = (R)k.vx, vy = (R)k.vy, vz = (R)k.vz;
-    const R rhoRef = (R)C.rhoRef, D1 = (R)C.D1;
+    const R rhoRef = (R)Cp->rhoRef, D1 = (R)Cp->D1;
+    const double dnT = (double)nT;
     for (int ti = item.tile0; ti < item.tile0 + item.ntile; ti++) {
       const Tile tl = tiles[ti];
       for (int k0 = 0; k0 < nT; k0 += tl.K) {
@@ -209,79 +199,111 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
         const int km0 = (k0 > 0) ? k0 - 1 : nT - 1;
         int base;
         {
-          const R* q = rec + (tl.e0 * nT + km0) * kRecFields;
-          const double r = retarded_r(k, ox - (double)q[0], oy - (double)q[1], oz - (double)q[2]);
-          double tb = obs_time(k, km0, r);
-          if (k0 == 0) tb = tb - k.lpi;
-          base = (int)floor(tb * k.inv_dPsi) - tl.spread - 2;
+          double r;
+          double tb = tau_at(k, load_rec<Rec>(rec + tl.e0 * nT + km0), km0, ox, oy, oz, &r);
+          if (k0 == 0) tb = tb - dnT;
+          base = (int)floor(tb) - tl.spread - 3;
         }
         int smin = W, smax = -1;
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
-          const R* re = rec + e * nT * kRecFields;
-          double t_prev, t_cur, r_cur, dxc, dyc, dzc;
-          {
-            const R* q = re + km0 * kRecFields;
-            const double r = retarded_r(k, ox - (double)q[0], oy - (double)q[1], oz - (double)q[2]);
-            t_prev = obs_time(k, km0, r);
-            if (k0 == 0) t_prev = t_prev - k.lpi;
-            const R* q0 = re + k0 * kRecFields;
-            dxc = ox - (double)q0[0];
-            dyc = oy - (double)q0[1];
-            dzc = oz - (double)q0[2];
-            r_cur = retarded_r(k, dxc, dyc, dzc);
-            t_cur = obs_time(k, k0, r_cur);
-          }
+          cptr<Rec> re = rec + e * nT;
+          double r_cur, r_tmp;
+          double t_prev = tau_at(k, load_rec<Rec>(re + km0), km0, ox, oy, oz, &r_tmp);
+          if (k0 == 0) t_prev = t_prev - dnT;
+          // two-deep record pipeline: qc = step kk (integrand), qn = step kk+1 (retarded solve),
+          // qnn = step kk+2 in flight
+          Rec qc = load_rec<Rec>(re + k0);
+          const int kn0 = (k0 + 1 < nT) ? k0 + 1 : 0;
+          Rec qn = load_rec<Rec>(re + kn0);
+          double t_cur = tau_at(k, qc, k0, ox, oy, oz, &r_cur);
           for (int kk = k0; kk < kend; kk++) {
             const int kn = (kk + 1 < nT) ? kk + 1 : 0;
-            const R* qn = re + kn * kRecFields;
-            const double dxn = ox - (double)qn[0], dyn = oy - (double)qn[1], dzn = oz - (double)qn[2];
-            const double r_next = retarded_r(k, dxn, dyn, dzn);
-            double t_next = obs_time(k, kn, r_next);
-            if (kk == nT - 1) t_next = t_next + k.lpi;
+            const int knn = (kn + 1 < nT) ? kn + 1 : 0;
+            const Rec qnn = load_rec<Rec>(re + knn);
+            // the rising side of this eval's tent is known before the solve: start the window
+            // reads of its footprint now
+            const double fc = floor(t_cur);
+            int sb = (int)fc - 1 - base;
+            const bool inrange = sb >= 0 && sb + 3 < W;
+            sb = inrange ? sb : (sb < 0 ? 0 : W - 4);
+            R2* p = &win[sb * 64 + lane];
+            R2 v0 = p[0], v1 = p[64], v2 = p[128], v3 = p[192];
             R pth, plo;
             bool bad;
-            integrand<R>(re + kk * kRecFields, (R)dxc, (R)dyc, (R)dzc, (R)r_cur, inv_a0, a0, vx, vy,
-                         vz, rhoRef, D1, &pth, &plo, &bad);
-            if (bad) flags |= kFlagNumeric;
-            const R idm = rcpT((R)(t_prev - t_cur));
+            integrand<R, Rec>(qc, oxr, oyr, ozr, (R)r_cur, inv_a0, a0, vx, vy, vz, rhoRef, D1,
+                              &pth, &plo, &bad);
+            const double r_next =
+                retarded_r(k, ox - (double)qn.x[0], oy - (double)qn.x[1], oz - (double)qn.x[2]);
+            double t_next = obs_tau(k, kn, r_next);
+            if (kk == nT - 1) t_next = t_next + dnT;
+            Tent T;
+            const bool ok = tent_of(t_prev, t_cur, t_next, &T) && !bad;
+            if (!ok) flags |= kFlagNumeric;
+            const R pthv = ok ? pth : (R)0, plov = ok ? plo : (R)0;
+            const R idm = rcpT((R)(t_cur - t_prev));
             const R idp = rcpT((R)(t_next - t_cur));
-            int ip, np, nd, nu;
-            auto put = [&](int a, R w) {
-              const int slot = a - base;
-              if (slot >= 0 && slot < W) {
-                __hip_atomic_fetch_add(&winT[slot * 64 + lane], pth * w, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                __hip_atomic_fetch_add(&winL[slot * 64 + lane], plo * w, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
-                smin = min(smin, slot);
-                smax = max(smax, slot);
-              } else {
-                flags |= kFlagWindow;
+            // footprint slots ic-1 .. ic+2: rising bins ic-1 (nd >= 2), ic (nd >= 1);
+            // falling bins ic+1 (nu >= 1), ic+2 (nu >= 2)
+            const R w0 = T.nd >= 2 ? (R)((fc - 1.0) - t_prev) * idm : (R)0;
+            const R w1 = T.nd >= 1 ? (R)(fc - t_prev) * idm : (R)0;
+            const R w2 = T.nu >= 1 ? (R)(t_next - (fc + 1.0)) * idp : (R)0;
+            const R w3 = T.nu >= 2 ? (R)(t_next - (fc + 2.0)) * idp : (R)0;
+            const bool any_on = (T.nd | T.nu) != 0;
+            if (any_on && !inrange) flags |= kFlagWindow;
+            v0.x = fmaT(pthv, w0, v0.x);
+            v0.y = fmaT(plov, w0, v0.y);
+            v1.x = fmaT(pthv, w1, v1.x);
+            v1.y = fmaT(plov, w1, v1.y);
+            v2.x = fmaT(pthv, w2, v2.x);
+            v2.y = fmaT(plov, w2, v2.y);
+            v3.x = fmaT(pthv, w3, v3.x);
+            v3.y = fmaT(plov, w3, v3.y);
+            p[0] = v0;
+            p[64] = v1;
+            p[128] = v2;
+            p[192] = v3;
+            if (any_on) {
+              smin = min(smin, sb);
+              smax = max(smax, sb + 3);
+            }
+            const bool tail = T.nd > 2 || T.nu > 2;
+            if (wave_any(tail)) {  // tents wider than two bins a side
+              if (tail) {
+                if (T.nd > W || T.nu > W) flags |= kFlagSweep;
+                const int nd = min(T.nd, W), nu = min(T.nu, W);
+                auto put = [&](int b, R w) {
+                  const int slot = b - base;
+                  if (slot < 0 || slot >= W) {
+                    flags |= kFlagWindow;
+                    return;
+                  }
+                  R2 v = win[slot * 64 + lane];
+                  v.x = fmaT(pthv, w, v.x);
+                  v.y = fmaT(plov, w, v.y);
+                  win[slot * 64 + lane] = v;
+                  smin = min(smin, slot);
+                  smax = max(smax, slot);
+                };
+                for (int j = 2; j < nd; j++) put(T.ic - j, (R)((fc - (double)j) - t_prev) * idm);
+                for (int j = 2; j < nu; j++) put(T.ic + 1 + j, (R)(t_next - (fc + (double)(1 + j))) * idp);
               }
-            };
-            const bool ok = bin_sweeps(
-                k, t_prev, t_cur, t_next, W, &ip, &np, &nd, &nu,
-                [&](int a, double num) { put(a, (R)num * idm); },
-                [&](int a, double num) { put(a, (R)num * idp); });
-            if (!ok) flags |= kFlagSweep | kFlagNumeric;
+            }
             t_prev = t_cur;
             t_cur = t_next;
             r_cur = r_next;
-            dxc = dxn;
-            dyc = dyn;
-            dzc = dzn;
+            qc = qn;
+            qn = qnn;
           }
         }
         // add the window into this item's slab row and clear it
         for (int s = smin; s <= smax; s++) {
-          const R vT = winT[s * 64 + lane], vL = winL[s * 64 + lane];
-          winT[s * 64 + lane] = (R)0;
-          winL[s * 64 + lane] = (R)0;
+          const R2 v = win[s * 64 + lane];
+          win[s * 64 + lane] = R2{(R)0, (R)0};
           if (active) {
             int b = (base + s) % nT;
             if (b < 0) b += nT;
-            slab[2 * b] += vT;
-            slab[2 * b + 1] += vL;
+            slab[2 * b] += v.x;
+            slab[2 * b + 1] += v.y;
           }
         }
       }
@@ -335,26 +357,32 @@ __global__ __launch_bounds__(256) void tonal_reduce_kernel(const TonalCaseDev* _
 }
 
 // Reference AoS (X[4], N[6], V[6], PR[2] per element-step, surfaces concatenated) -> records.
-template <typename R>
+template <typename Rec, typename H>
 __global__ void tonal_prep_kernel(const float* __restrict__ X, const float* __restrict__ N,
                                   const float* __restrict__ V, const float* __restrict__ PR,
-                                  int64_t n, double inv_a0, R* __restrict__ rec) {
+                                  int64_t n, double inv_a0, Rec* __restrict__ rec) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
     const float* x = X + 4 * q;
     const float* nn = N + 6 * q;
     const float* v = V + 6 * q;
     const float* p = PR + 2 * q;
-    R* o = rec + q * kRecFields;
-    for (int i = 0; i < 4; i++) o[i] = (R)x[i];
-    for (int i = 0; i < 6; i++) o[4 + i] = (R)v[i];
-    for (int i = 0; i < 6; i++) o[10 + i] = (R)nn[i];
-    o[16] = (R)p[0];
-    o[17] = (R)p[1];
+    Rec o;
+    for (int i = 0; i < 3; i++) {
+      o.x[i] = x[i];
+      o.v[i] = v[i];
+      o.vd[i] = v[3 + i];
+      o.n[i] = nn[i];
+      o.nd[i] = nn[3 + i];
+    }
+    o.ds = x[3];
+    o.P = p[0];
+    o.dP = p[1];
     const double vv = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
     const double M = sqrt(vv) * inv_a0;
-    o[18] = (R)(M * M);
-    o[19] = (R)((double)v[0] * nn[0] + (double)v[1] * nn[1] + (double)v[2] * nn[2]);
+    o.MM = (H)(M * M);
+    o.Vn = (H)((double)v[0] * nn[0] + (double)v[1] * nn[1] + (double)v[2] * nn[2]);
+    rec[q] = o;
   }
 }
 
@@ -370,30 +398,40 @@ __global__ void reset_counter_kernel(int* c, uint32_t* flags) {
   *flags = 0u;
 }
 
-// Bin lookup of every (element, step) for one observer, through the same sweep code.
-template <typename R>
-__global__ void tonal_bins_kernel(const R* __restrict__ rec, int64_t nE, TimeConst k, double ox,
+// Tent of every (element, step) for one observer, through the same tent code: the top rising
+// bin ic (stored as ic mod nT, with its period index floor(ic / nT)) and the two side lengths.
+template <typename Rec>
+__global__ void tonal_bins_kernel(const Rec* __restrict__ rec, int64_t nE, TimeConst k, double ox,
                                   double oy, double oz, int32_t* ipts, int32_t* npi, int32_t* nd,
                                   int32_t* nu) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= nE) return;
   const int nT = k.nT;
-  const R* re = rec + e * nT * kRecFields;
-  auto tob = [&](int j) {
-    const R* q = re + j * kRecFields;
-    return obs_time(k, j, retarded_r(k, ox - (double)q[0], oy - (double)q[1], oz - (double)q[2]));
-  };
-  double t_prev = tob(nT - 1) - k.lpi, t_cur = tob(0);
+  const Rec* re = rec + e * nT;
+  double r;
+  double t_prev = tau_at(k, re[nT - 1], nT - 1, ox, oy, oz, &r) - (double)nT;
+  double t_cur = tau_at(k, re[0], 0, ox, oy, oz, &r);
   for (int kk = 0; kk < nT; kk++) {
-    double t_next = (kk + 1 < nT) ? tob(kk + 1) : tob(0) + k.lpi;
-    int a, b, c, d;
-    const bool ok = bin_sweeps(k, t_prev, t_cur, t_next, 1 << 20, &a, &b, &c, &d,
-                               [](int, double) {}, [](int, double) {});
+    double t_next = (kk + 1 < nT) ? tau_at(k, re[kk + 1], kk + 1, ox, oy, oz, &r)
+                                  : tau_at(k, re[0], 0, ox, oy, oz, &r) + (double)nT;
     const int64_t idx = e * nT + kk;
-    ipts[idx] = ok ? a : -1;
-    npi[idx] = b;
-    nd[idx] = c;
-    nu[idx] = d;
+    Tent T;
+    if (tent_of(t_prev, t_cur, t_next, &T)) {
+      int per = T.ic / nT, ph = T.ic % nT;
+      if (ph < 0) {
+        ph += nT;
+        per -= 1;
+      }
+      ipts[idx] = ph;
+      npi[idx] = per;
+      nd[idx] = T.nd;
+      nu[idx] = T.nu;
+    } else {
+      ipts[idx] = -1;
+      npi[idx] = 0;
+      nd[idx] = -1;
+      nu[idx] = -1;
+    }
     t_prev = t_cur;
     t_cur = t_next;
   }
@@ -412,6 +450,7 @@ hipError_t launch_tonal_eval(bool mixed, int W, int grid, hipStream_t s, const T
     return hipGetLastError();                                                                \
   }
   ACUM_EVAL(true, 32)
+  ACUM_EVAL(true, 48)
   ACUM_EVAL(true, 64)
   ACUM_EVAL(true, 128)
   ACUM_EVAL(false, 32)
@@ -426,6 +465,7 @@ hipError_t eval_occupancy(bool mixed, int W, int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                       \
                                                         tonal_eval_kernel<M, WW>, 64, 0);
   ACUM_OCC(true, 32)
+  ACUM_OCC(true, 48)
   ACUM_OCC(true, 64)
   ACUM_OCC(true, 128)
   ACUM_OCC(false, 32)
@@ -448,11 +488,11 @@ hipError_t launch_tonal_prep(bool mixed, hipStream_t s, const float* X, const fl
                              const float* V, const float* PR, int64_t n, double inv_a0, void* rec) {
   const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
   if (mixed)
-    hipLaunchKernelGGL(tonal_prep_kernel<float>, dim3(grid), dim3(256), 0, s, X, N, V, PR, n, inv_a0,
-                       (float*)rec);
+    hipLaunchKernelGGL((tonal_prep_kernel<Rec32, float>), dim3(grid), dim3(256), 0, s, X, N, V, PR, n,
+                       inv_a0, (Rec32*)rec);
   else
-    hipLaunchKernelGGL(tonal_prep_kernel<double>, dim3(grid), dim3(256), 0, s, X, N, V, PR, n, inv_a0,
-                       (double*)rec);
+    hipLaunchKernelGGL((tonal_prep_kernel<Rec64, double>), dim3(grid), dim3(256), 0, s, X, N, V, PR, n,
+                       inv_a0, (Rec64*)rec);
   return hipGetLastError();
 }
 
@@ -477,10 +517,10 @@ hipError_t launch_tonal_bins(bool mixed, hipStream_t s, const void* rec, int64_t
                              int32_t* nd, int32_t* nu) {
   const int grid = (int)((nE + 63) / 64);
   if (mixed)
-    hipLaunchKernelGGL(tonal_bins_kernel<float>, dim3(grid), dim3(64), 0, s, (const float*)rec, nE, k,
+    hipLaunchKernelGGL(tonal_bins_kernel<Rec32>, dim3(grid), dim3(64), 0, s, (const Rec32*)rec, nE, k,
                        ox, oy, oz, ipts, npi, nd, nu);
   else
-    hipLaunchKernelGGL(tonal_bins_kernel<double>, dim3(grid), dim3(64), 0, s, (const double*)rec, nE,
+    hipLaunchKernelGGL(tonal_bins_kernel<Rec64>, dim3(grid), dim3(64), 0, s, (const Rec64*)rec, nE,
                        k, ox, oy, oz, ipts, npi, nd, nu);
   return hipGetLastError();
 }

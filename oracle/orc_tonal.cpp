@@ -10,16 +10,22 @@
 // Summation order (the reference's is undefined: unordered float atomics, core.cu:202-226):
 //   per observer -> surface -> element -> source step, each bin accumulated in that order.
 //
-// Observer time and bin lookup are written in the canonical FP64 form shared with the GPU
-// kernels (same operations, same order, no contraction), so bin indices must agree exactly:
-//   a = (|vinf|^2) - a0^2, four_a = 4a, inv_2a = 1/(2a)                  (core.cu:265)
-//   d = obs - src; c = |d|^2; b = -2 (d.vinf); disc = b^2 - four_a*c      (core.cu:266-268)
-//   t = (-b - sqrt(disc)) inv_2a, or (-b + sqrt(disc)) inv_2a if negative  (core.cu:273-276)
-//   r = t a0;  tobs(j) = (j dTau + r inv_a0) oM                            (core.cu:156,190,194)
-//   npi = floor((lpi - tobs) inv_lpi); newx = tobs + npi lpi               (core.cu:298-300)
-//   ipts = largest k in [0,nT) with k dPsi <= newx                        (core.cu:325-355)
-// Deviation, documented: where the reference's bsearch would return -1 (newx < 0, undefined
-// behaviour at core.cu:197), npi is raised by one so that newx lies in [0, lpi].
+// Observer time and the tent are written in the canonical FP64 form shared with the GPU
+// kernels (same operations, same order, explicit fma, no other contraction), so bin indices
+// agree exactly.  Time is measured in observer-time bins (units of dPsi):
+//   d = obs - src; c = fma(dz,dz, fma(dy,dy, dx*dx))                      (core.cu:267)
+//   still air: r = sqrt(c); otherwise a = |vinf|^2 - a0^2, b = -2 fma(dz,vz, fma(dy,vy, dx*vx)),
+//   disc = fma(b, b, -(4a c)), t = (-b - sqrt(disc)) / (2a), or the + root if negative, r = t a0
+//                                                                          (core.cu:261-279)
+//   tau(j) = fma(r, (oM/a0)/dPsi, j) = (j dTau + r/a0) oM / dPsi          (core.cu:156)
+// findpts/bsearch (core.cu:290-355) pick the grid time at or below the shifted observer time
+// and the two sweeps (core.cu:196-231) visit the grid times in (ttem, ttemp] (weights rising
+// from 0 to 1) and in (ttemp, ttep) (weights falling from 1 to 0), wrapping periodically.  In
+// bin units the grid times are the integers: the rising bins are the integers b in
+// (tau_prev, tau] with weight (b - tau_prev)/(tau - tau_prev), the falling bins the integers in
+// (tau, tau_next) with weight (tau_next - b)/(tau_next - tau), and bin b is stored at b mod nT.
+// The reference's radian-domain steps (npi shift, repeated tObs2 -= dPsi) select the same bins
+// in exact arithmetic; tests check the two forms agree on sampled cases.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -30,23 +36,17 @@
 namespace {
 
 struct TimeConst {
-  double a0, inv_a0, dTau, oM, lpi, inv_lpi, dPsi, inv_dPsi, four_a, inv_2a;
+  double a0, r2b, four_a, inv_2a;
   double vx, vy, vz;
   int64_t nT;
-  bool still;  // vinf == 0: b vanishes exactly, same bits as the general form
+  bool still;  // vinf == 0
 };
 
 TimeConst make_time_const(const acum_tonal_case* c) {
   TimeConst k;
   k.a0 = c->a0;
-  k.inv_a0 = 1.0 / c->a0;
-  k.dTau = c->dTau;
-  k.oM = c->oM;
+  k.r2b = (c->oM / c->a0) / c->dPsi;
   k.nT = c->nTime;
-  k.dPsi = c->dPsi;
-  k.inv_dPsi = 1.0 / c->dPsi;
-  k.lpi = (double)c->nTime * c->dPsi;
-  k.inv_lpi = 1.0 / k.lpi;
   k.vx = c->vInf[0];
   k.vy = c->vInf[1];
   k.vz = c->vInf[2];
@@ -60,57 +60,26 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   return k;
 }
 
-// Convected retarded distance r (acous_dist, core.cu:261-279) in the canonical FP64 form.
+// Convected retarded distance r (acous_dist, core.cu:261-279): the smallest positive root of
+// (|vinf|^2 - a0^2) t^2 - 2 (d.vinf) t + |d|^2 = 0, times a0; r = |d| in still air.
 inline double retarded_r(const TimeConst& k, double ox, double oy, double oz,
                          const float* src) {
-  double dx = ox - (double)src[0];
-  double dy = oy - (double)src[1];
-  double dz = oz - (double)src[2];
-  double c = dx * dx;
-  c = c + dy * dy;
-  c = c + dz * dz;
-  double b = 0.0;
-  if (!k.still) {
-    double bh = dx * k.vx;
-    bh = bh + dy * k.vy;
-    bh = bh + dz * k.vz;
-    b = -2.0 * bh;
-  }
-  double disc = b * b - k.four_a * c;
-  double sq = std::sqrt(disc);
+  const double dx = ox - (double)src[0];
+  const double dy = oy - (double)src[1];
+  const double dz = oz - (double)src[2];
+  const double c = std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+  if (k.still) return std::sqrt(c);
+  const double b = -2.0 * std::fma(dz, k.vz, std::fma(dy, k.vy, dx * k.vx));
+  const double disc = std::fma(b, b, -(k.four_a * c));
+  const double sq = std::sqrt(disc);
   double t = (-b - sq) * k.inv_2a;
   if (t < 0.0) t = (-b + sq) * k.inv_2a;
   return t * k.a0;
 }
 
-inline double obs_time(const TimeConst& k, int64_t j, double r) {
-  return ((double)j * k.dTau + r * k.inv_a0) * k.oM;
-}
-
-// Bin lookup of one eval (findpts + bsearch, core.cu:169,290-355).  Returns false if the
-// observer time is not finite.
-inline bool find_bin(const TimeConst& k, double ttemp, int64_t* ipts_out, int64_t* npi_out,
-                     double* shift_out) {
-  if (!std::isfinite(ttemp)) return false;
-  double q = std::floor((k.lpi - ttemp) * k.inv_lpi);
-  int64_t npi = (int64_t)q;
-  double shift = (double)npi * k.lpi;
-  double newx = ttemp + shift;
-  if (newx < 0.0) {
-    npi += 1;
-    shift = (double)npi * k.lpi;
-    newx = ttemp + shift;
-  }
-  double f = std::floor(newx * k.inv_dPsi);
-  int64_t ip = (int64_t)f;
-  if (ip < 0) ip = 0;
-  if (ip > k.nT - 1) ip = k.nT - 1;
-  while (ip + 1 < k.nT && (double)(ip + 1) * k.dPsi <= newx) ip++;
-  while (ip > 0 && (double)ip * k.dPsi > newx) ip--;
-  *ipts_out = ip;
-  *npi_out = npi;
-  *shift_out = shift;
-  return true;
+// Observer time of step j in bins, (j dTau + r/a0) oM / dPsi (core.cu:156).
+inline double obs_tau(const TimeConst& k, int64_t j, double r) {
+  return std::fma(r, k.r2b, (double)j);
 }
 
 struct Eval {
@@ -164,45 +133,33 @@ inline void integrand(const float* X, const float* Nn, const float* Vv, const fl
   *pLoad = LN + LF1;                                                  // core.cu:154
 }
 
-// Runs the tent binning of one eval (core.cu:171-231) and calls add(bin, w) in the
-// reference's order: down-sweep bins first, then up-sweep bins.
+// Tent binning of one eval (core.cu:166-231) in bin units: add(bin, w) for every rising bin,
+// top down, then every falling bin, bottom up (the reference's order).  Returns false for a
+// non-finite observer time or a tent longer than the period.
 template <typename Add>
-inline bool bin_eval(const TimeConst& k, double ttem, double ttemp, double ttep, int64_t* nd,
-                     int64_t* nu, int64_t* ipts_o, int64_t* npi_o, Add add) {
-  int64_t ipts, npi;
-  double shift;
-  if (!find_bin(k, ttemp, &ipts, &npi, &shift)) return false;
+inline bool bin_eval(const TimeConst& k, double tp, double tc, double tn, int64_t* nd, int64_t* nu,
+                     int64_t* ic_o, Add add) {
+  if (!std::isfinite(tp) || !std::isfinite(tc) || !std::isfinite(tn)) return false;
   const int64_t nT = k.nT;
-  const double ttem_s = ttem + shift;
-  const double dm = ttem - ttemp;
-  int64_t iptm = ipts, cnt = 0;
-  double tObs2 = (double)ipts * k.dPsi;
-  while (tObs2 > ttem_s) {
-    double w = (ttem_s - tObs2) / dm;
-    add(iptm, w);
-    tObs2 -= k.dPsi;
-    iptm -= 1;
-    if (iptm < 0) iptm = nT - 1;
+  const double fc = std::floor(tc);
+  const int64_t ic = (int64_t)fc;
+  auto phys = [nT](int64_t b) {
+    int64_t m = b % nT;
+    return m < 0 ? m + nT : m;
+  };
+  int64_t cnt = 0;
+  for (int64_t b = ic; (double)b > tp; b--) {  // rising side: (tau_prev, tau]
+    add(phys(b), ((double)b - tp) / (tc - tp));
     if (++cnt > nT) return false;
   }
   *nd = cnt;
-  int64_t iptp = (ipts < nT - 1) ? ipts + 1 : 0;
-  tObs2 = (double)iptp * k.dPsi;
-  if (iptp == 0) tObs2 += k.lpi;
-  const double ttep_s = ttep + shift;
-  const double dp = ttep - ttemp;
   cnt = 0;
-  while (tObs2 < ttep_s) {
-    double w = (ttep_s - tObs2) / dp;
-    add(iptp, w);
-    tObs2 += k.dPsi;
-    iptp += 1;
-    if (iptp > nT - 1) iptp = 0;
+  for (int64_t b = ic + 1; (double)b < tn; b++) {  // falling side: (tau, tau_next)
+    add(phys(b), (tn - (double)b) / (tn - tc));
     if (++cnt > nT) return false;
   }
   *nu = cnt;
-  *ipts_o = ipts;
-  *npi_o = npi;
+  *ic_o = ic;
   return true;
 }
 
@@ -236,16 +193,16 @@ int tonal_impl(const acum_tonal_case* c, Out* pT, Out* pL, Out* pA, int nthreads
           for (int64_t t = 0; t < nT && ok; t++) {
             const int64_t mm = (t > 0) ? t - 1 : nT - 1;       // core.cu:173-179
             const int64_t mp = (t < nT - 1) ? t + 1 : 0;       // core.cu:180-186
-            const double ttemp = obs_time(k, t, rr[t]);
-            double ttem = obs_time(k, mm, rr[mm]);
-            if (t == 0) ttem = ttem - k.lpi;
-            double ttep = obs_time(k, mp, rr[mp]);
-            if (t == nT - 1) ttep = ttep + k.lpi;
+            const double tc = obs_tau(k, t, rr[t]);
+            double tp = obs_tau(k, mm, rr[mm]);
+            if (t == 0) tp = tp - (double)nT;  // previous step of the periodic history
+            double tn = obs_tau(k, mp, rr[mp]);
+            if (t == nT - 1) tn = tn + (double)nT;
             R pth, plo;
             integrand<R>(Xe + 4 * t, Ne + 6 * t, Ve + 6 * t, Pe + 2 * t, ox, oy, oz, rr[t], k,
                          (R)c->rhoRef, &pth, &plo);
-            int64_t nd, nu, ip, np;
-            ok = bin_eval(k, ttem, ttemp, ttep, &nd, &nu, &ip, &np, [&](int64_t b, double w) {
+            int64_t nd, nu, ic;
+            ok = bin_eval(k, tp, tc, tn, &nd, &nu, &ic, [&](int64_t b, double w) {
               const R wr = (R)w;
               bT[b] += pth * wr;
               bL[b] += plo * wr;
@@ -280,7 +237,8 @@ int orc_tonal_f32(const acum_tonal_case* c, float* pT, float* pL, float* pA, int
   return tonal_impl<float, float>(c, pT, pL, pA, nthreads);
 }
 
-// Bin lookup of every eval of one observer, in case order (surface, element, step).
+// Tent of every eval of one observer, in case order (surface, element, step): the top rising
+// bin as (ic mod nT, floor(ic / nT)) and the two side lengths.
 int orc_tonal_bins(const acum_tonal_case* c, int64_t iobs, int32_t* ipts, int32_t* npi,
                    int32_t* ndown, int32_t* nup) {
   if (!c || iobs < 0 || iobs >= c->nObs) return ACUM_ERR_ARG;
@@ -296,16 +254,20 @@ int orc_tonal_bins(const acum_tonal_case* c, int64_t iobs, int32_t* ipts, int32_
       for (int64_t t = 0; t < nT; t++, idx++) {
         const int64_t mm = (t > 0) ? t - 1 : nT - 1;
         const int64_t mp = (t < nT - 1) ? t + 1 : 0;
-        const double ttemp = obs_time(k, t, rr[t]);
-        double ttem = obs_time(k, mm, rr[mm]);
-        if (t == 0) ttem = ttem - k.lpi;
-        double ttep = obs_time(k, mp, rr[mp]);
-        if (t == nT - 1) ttep = ttep + k.lpi;
-        int64_t nd = -1, nu = -1, ip = -1, np = 0;
-        if (!bin_eval(k, ttem, ttemp, ttep, &nd, &nu, &ip, &np, [](int64_t, double) {}))
-          return ACUM_ERR_NUMERIC;
-        ipts[idx] = (int32_t)ip;
-        npi[idx] = (int32_t)np;
+        const double tc = obs_tau(k, t, rr[t]);
+        double tp = obs_tau(k, mm, rr[mm]);
+        if (t == 0) tp = tp - (double)nT;
+        double tn = obs_tau(k, mp, rr[mp]);
+        if (t == nT - 1) tn = tn + (double)nT;
+        int64_t nd = -1, nu = -1, ic = 0;
+        if (!bin_eval(k, tp, tc, tn, &nd, &nu, &ic, [](int64_t, double) {})) return ACUM_ERR_NUMERIC;
+        int64_t per = ic / nT, ph = ic % nT;
+        if (ph < 0) {
+          ph += nT;
+          per -= 1;
+        }
+        ipts[idx] = (int32_t)ph;
+        npi[idx] = (int32_t)per;
         ndown[idx] = (int32_t)nd;
         nup[idx] = (int32_t)nu;
       }
@@ -314,11 +276,11 @@ int orc_tonal_bins(const acum_tonal_case* c, int64_t iobs, int32_t* ipts, int32_
   return ACUM_OK;
 }
 
-// Observer time of one (element position, step) in the canonical FP64 form (for tests).
-double orc_obs_time(const acum_tonal_case* c, const float* src, const float* obs, int64_t j) {
+// Observer time (bins) of one (element position, step) in the canonical FP64 form (for tests).
+double orc_obs_tau(const acum_tonal_case* c, const float* src, const float* obs, int64_t j) {
   const TimeConst k = make_time_const(c);
   double r = retarded_r(k, obs[0], obs[1], obs[2], src);
-  return obs_time(k, j, r);
+  return obs_tau(k, j, r);
 }
 
 double orc_retarded_r(const acum_tonal_case* c, const float* src, const float* obs) {

@@ -25,7 +25,7 @@ def _load_oracle():
         "orc_tonal_f64": (C.c_int, [tc, _f64p, _f64p, _f64p, C.c_int]),
         "orc_tonal_f32": (C.c_int, [tc, _f32p, _f32p, _f32p, C.c_int]),
         "orc_tonal_bins": (C.c_int, [tc, C.c_int64, _i32p, _i32p, _i32p, _i32p]),
-        "orc_obs_time": (C.c_double, [tc, _f32p, _f32p, C.c_int64]),
+        "orc_obs_tau": (C.c_double, [tc, _f32p, _f32p, C.c_int64]),
         "orc_retarded_r": (C.c_double, [tc, _f32p, _f32p]),
         "orc_differentiate_f32": (None, [_f32p, _f32p, C.c_float, C.c_int]),
         "orc_differentiate_biased_f32": (None, [_f32p, _f32p, C.c_float, C.c_int]),
