@@ -13,18 +13,21 @@
 #include "abi_internal.hpp"
 
 namespace acum {
-hipError_t launch_tonal_eval(bool mixed, int W, int grid, hipStream_t s, const TonalCaseDev* cases,
-                             const Tile* tiles, const Item* items, int nItems, int* counter,
-                             uint32_t* flags);
-hipError_t eval_occupancy(bool mixed, int W, int* blocks_per_cu);
+hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
+                             const TonalCaseDev* cases, const Tile* tiles, const Item* items,
+                             int nItems, int* counter, uint32_t* flags);
+hipError_t eval_occupancy(bool mixed, bool moving, int W, int* blocks_per_cu);
+bool eval_window_supported(bool mixed, int W);
 hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* cases, int icase,
                                int nT, int nObs);
-hipError_t launch_tonal_prep(bool mixed, hipStream_t s, const float* X, const float* N,
-                             const float* V, const float* PR, int64_t n, double inv_a0, void* rec);
+hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, int nT, double SB, void* pos);
+hipError_t launch_tonal_attr(bool mixed, hipStream_t s, const float* X, const float* N,
+                             const float* V, const float* PR, int64_t n, double a0, double rhoRef,
+                             void* attr);
 hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes);
 hipError_t launch_reset(hipStream_t s, int* counter, uint32_t* flags);
-hipError_t launch_tonal_bins(bool mixed, hipStream_t s, const void* rec, int64_t nE,
-                             const TimeConst& k, double ox, double oy, double oz, int32_t* ipts,
+hipError_t launch_tonal_bins(hipStream_t s, const void* pos, int64_t nE, const TimeConst& k,
+                             double ox, double oy, double oz, int32_t* ipts,
                              int32_t* npi, int32_t* nd, int32_t* nu);
 }  // namespace acum
 
@@ -36,12 +39,13 @@ namespace {
 struct StagedCase {
   int64_t nE = 0, nT = 0, nObs = 0;
   TimeConst k{};
-  double rhoRef = 0, D1 = 0, Minf = 0;
+  double rhoRef = 0, Minf = 0;
   std::vector<int64_t> surf0;  // first element of each surface
   std::vector<float> pos;      // host positions [nE][nT][3], for planning
   float* d_obs = nullptr;
   float* d_aos[4] = {nullptr, nullptr, nullptr, nullptr};  // X, N, V, PR
-  void* d_rec[2] = {nullptr, nullptr};                      // [ACUM_FP64], [ACUM_MIXED]
+  PosRec* d_pos = nullptr;                                  // shared by both precisions
+  void* d_attr[2] = {nullptr, nullptr};                     // [ACUM_FP64], [ACUM_MIXED]
   float* d_out[3] = {nullptr, nullptr, nullptr};
 };
 
@@ -57,7 +61,8 @@ struct Plan {
   Item* d_items = nullptr;
   TonalCaseDev* d_cases = nullptr;
   void* d_slab = nullptr;
-  int grid = 0;
+  int n_still = 0;  // items[0, n_still) are still-air cases, the rest moving-medium ones
+  int grid[2] = {0, 0};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -121,7 +126,8 @@ void free_stage(Stage& s) {
   for (auto& c : s.cases) {
     (void)hipFree(c.d_obs);
     for (auto& p : c.d_aos) (void)hipFree(p);
-    for (auto& p : c.d_rec) (void)hipFree(p);
+    (void)hipFree(c.d_pos);
+    for (auto& p : c.d_attr) (void)hipFree(p);
     for (auto& p : c.d_out) (void)hipFree(p);
   }
   for (auto& p : s.plan) free_plan(p);
@@ -133,6 +139,8 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   k.a0 = c->a0;
   k.inv_a0 = 1.0 / c->a0;
   k.r2b = (c->oM / c->a0) / c->dPsi;
+  k.KB = c->oM / c->dPsi;
+  k.SB = c->dTau * k.KB;
   k.nT = (int32_t)c->nTime;
   k.vx = c->vInf[0];
   k.vy = c->vInf[1];
@@ -140,9 +148,8 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   double vv = k.vx * k.vx;
   vv = vv + k.vy * k.vy;
   vv = vv + k.vz * k.vz;
-  const double a = vv - k.a0 * k.a0;
-  k.four_a = 4.0 * a;
-  k.inv_2a = 1.0 / (2.0 * a);
+  k.na = k.a0 * k.a0 - vv;
+  k.inv_na = 1.0 / k.na;
   k.still = (k.vx == 0.0 && k.vy == 0.0 && k.vz == 0.0) ? 1 : 0;
   return k;
 }
@@ -176,7 +183,6 @@ int stage_case(acum_ctx* ctx, const acum_tonal_case* c, StagedCase& sc) {
   sc.nObs = c->nObs;
   sc.k = make_time_const(c);
   sc.rhoRef = c->rhoRef;
-  sc.D1 = 1.0 / (4.0 * M_PI * c->a0);
   sc.Minf = std::sqrt(c->vInf[0] * c->vInf[0] + c->vInf[1] * c->vInf[1] + c->vInf[2] * c->vInf[2]) / c->a0;
   sc.surf0.resize(c->nSurf);
   int64_t nE = 0;
@@ -219,14 +225,19 @@ int stage_case(acum_ctx* ctx, const acum_tonal_case* c, StagedCase& sc) {
 }
 
 int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
-  if (sc.d_rec[prec]) return ACUM_OK;
+  if (sc.d_attr[prec]) return ACUM_OK;
   if (!sc.d_aos[0]) return ctx_fail(ctx, ACUM_ERR_ARG, "staged element data already released");
   const bool mixed = prec == ACUM_MIXED;
-  const size_t bytes = (size_t)sc.nE * sc.nT * (mixed ? sizeof(Rec32) : sizeof(Rec64));
-  HIPCHK(ctx, hipMalloc(&sc.d_rec[prec], bytes));
-  HIPCHK(ctx, launch_tonal_prep(mixed, ctx->stream, sc.d_aos[0], sc.d_aos[1], sc.d_aos[2], sc.d_aos[3],
-                                sc.nE * sc.nT, sc.k.inv_a0, sc.d_rec[prec]));
-  if (sc.d_rec[0] && sc.d_rec[1]) {  // both record sets built: the AoS copy is no longer needed
+  const int64_t n = sc.nE * sc.nT;
+  if (!sc.d_pos) {
+    HIPCHK(ctx, hipMalloc(&sc.d_pos, (size_t)n * sizeof(PosRec)));
+    HIPCHK(ctx, launch_tonal_pos(ctx->stream, sc.d_aos[0], n, (int)sc.nT, sc.k.SB, sc.d_pos));
+  }
+  const size_t bytes = (size_t)n * (mixed ? sizeof(AttrT<float>) : sizeof(AttrT<double>));
+  HIPCHK(ctx, hipMalloc(&sc.d_attr[prec], bytes));
+  HIPCHK(ctx, launch_tonal_attr(mixed, ctx->stream, sc.d_aos[0], sc.d_aos[1], sc.d_aos[2], sc.d_aos[3],
+                                n, sc.k.a0, sc.rhoRef, sc.d_attr[prec]));
+  if (sc.d_attr[0] && sc.d_attr[1]) {  // both record sets built: the AoS copy is no longer needed
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     for (auto& p : sc.d_aos) {
       (void)hipFree(p);
@@ -236,11 +247,14 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
   return ACUM_OK;
 }
 
-// Tiles of one case: runs of consecutive elements of one surface whose positions stay within
-// `max_spread` observer-time bins of the first element's at every step (Lipschitz bound of the
-// convected retarded distance: |dr| <= |dx| / (1 - Minf)), with the pass length K chosen so the
-// bins of a pass, with a 3-bin margin each side for the 4-slot footprint, fit the window:
-// (K + 1) * step_bins + 2 * spread + 8 <= W.
+// Tiles of one case: runs of consecutive elements of one surface whose observer times stay within
+// `spread` - 1 bins of the first element's at every step (Lipschitz bound of the convected
+// retarded time: |dtau| <= |dx| L, L = oM / (a0 dPsi (1 - Minf)) bins per metre), with the pass
+// length K chosen so every bin a pass can touch fits the window.  With the window based at
+// floor(tau_e0(k0 - 1)) - spread - 1 and at most step_bins bins between consecutive steps:
+//   lowest bin  >= floor(tau_e(k0)) - 1 >= base                       (footprint ic - 1)
+//   highest bin <= tau_e0(k0 - 1) + max(K step_bins + 2, (K + 1) step_bins) + spread
+// so K step_bins + 2 spread + 5 <= W and (K + 1) step_bins + 2 spread + 3 <= W.
 int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vector<Tile>& tiles) {
   const int64_t nT = sc.nT;
   const double L = sc.k.r2b / (1.0 - sc.Minf);  // bins per metre
@@ -260,8 +274,9 @@ int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vec
     step[e] = std::sqrt(m);
   }
   auto pass_len = [&](int spread, double S) {
-    const double sb = 1.0 + S * L;
-    const int K = (int)std::floor((W - 2.0 * spread - 8.0) / sb) - 1;
+    const double sb = sc.k.SB + S * L;
+    const int K = (int)std::min(std::floor((W - 2.0 * spread - 5.0) / sb),
+                                std::floor((W - 2.0 * spread - 3.0) / sb) - 1);
     return std::min<int64_t>(std::min<int64_t>(K, t.max_chunk), nT);
   };
   const size_t nsurf = sc.surf0.size();
@@ -314,8 +329,12 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
   free_plan(p);
   p.tune = ctx->tune[prec];
   const bool mixed = prec == ACUM_MIXED;
+  if (!eval_window_supported(mixed, p.tune.window))
+    return ctx_fail(ctx, ACUM_ERR_ARG, "no eval kernel for this window size");
+  bool any_moving = false;
+  for (auto& sc : st.cases) any_moving = any_moving || !sc.k.still;
   int bpc = 0;
-  HIPCHK(ctx, eval_occupancy(mixed, p.tune.window, &bpc));
+  HIPCHK(ctx, eval_occupancy(mixed, any_moving, p.tune.window, &bpc));
   if (bpc < 1) return ctx_fail(ctx, ACUM_ERR_DEVICE, "eval kernel does not fit a compute unit");
   const int capacity = bpc * ctx->cu_count;
   const int nc = (int)st.cases.size();
@@ -375,8 +394,14 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
       t0 = t1;
     }
   }
-  std::stable_sort(p.items.begin(), p.items.end(),
-                   [](const Item& a, const Item& b) { return a.cost > b.cost; });
+  // still-air items first (one kernel), then moving-medium items (another); longest first in each
+  std::stable_sort(p.items.begin(), p.items.end(), [&](const Item& a, const Item& b) {
+    const int sa = st.cases[a.icase].k.still, sb = st.cases[b.icase].k.still;
+    if (sa != sb) return sa > sb;
+    return a.cost > b.cost;
+  });
+  p.n_still = 0;
+  for (const Item& it : p.items) p.n_still += st.cases[it.icase].k.still ? 1 : 0;
   HIPCHK(ctx, hipMalloc(&p.d_tiles, p.tiles.size() * sizeof(Tile)));
   HIPCHK(ctx, hipMalloc(&p.d_items, p.items.size() * sizeof(Item)));
   HIPCHK(ctx, hipMalloc(&p.d_cases, nc * sizeof(TonalCaseDev)));
@@ -385,7 +410,8 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
   for (int c = 0; c < nc; c++) {
     const StagedCase& sc = st.cases[c];
     TonalCaseDev& d = hc[c];
-    d.rec = sc.d_rec[prec];
+    d.pos = sc.d_pos;
+    d.attr = sc.d_attr[prec];
     d.obs = sc.d_obs;
     d.slab = (char*)p.d_slab + p.slab_off[c];
     d.pT = sc.d_out[0];
@@ -395,14 +421,13 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     d.nObs = (int32_t)sc.nObs;
     d.G = p.G[c];
     d.k = sc.k;
-    d.rhoRef = sc.rhoRef;
-    d.D1 = sc.D1;
   }
   HIPCHK(ctx, hipMemcpyAsync(p.d_tiles, p.tiles.data(), p.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(p.d_items, p.items.data(), p.items.size() * sizeof(Item), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(p.d_cases, hc.data(), nc * sizeof(TonalCaseDev), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  p.grid = std::min<int>((int)p.items.size(), capacity);
+  p.grid[0] = std::min<int>(p.n_still, capacity);
+  p.grid[1] = std::min<int>((int)p.items.size() - p.n_still, capacity);
   p.valid = true;
   return ACUM_OK;
 }
@@ -416,8 +441,13 @@ int enqueue_pass(acum_ctx* ctx, Stage& st, int prec, bool with_events) {
   HIPCHK(ctx, launch_reset(s, ctx->d_counter, ctx->d_flags));
   HIPCHK(ctx, launch_fill_zero(s, p.d_slab, p.slab_bytes));
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
-  HIPCHK(ctx, launch_tonal_eval(mixed, p.tune.window, p.grid, s, p.d_cases, p.d_tiles, p.d_items,
-                                (int)p.items.size(), ctx->d_counter, ctx->d_flags));
+  const int nmov = (int)p.items.size() - p.n_still;
+  if (p.n_still > 0)
+    HIPCHK(ctx, launch_tonal_eval(mixed, false, p.tune.window, p.grid[0], s, p.d_cases, p.d_tiles,
+                                  p.d_items, p.n_still, ctx->d_counter, ctx->d_flags));
+  if (nmov > 0)
+    HIPCHK(ctx, launch_tonal_eval(mixed, true, p.tune.window, p.grid[1], s, p.d_cases, p.d_tiles,
+                                  p.d_items + p.n_still, nmov, ctx->d_counter + 1, ctx->d_flags));
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
   for (int c = 0; c < (int)st.cases.size(); c++)
     HIPCHK(ctx, launch_tonal_reduce(mixed, s, p.d_cases, c, (int)st.cases[c].nT, (int)st.cases[c].nObs));
@@ -517,11 +547,13 @@ int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
 
 acum_tune default_tune(int prec) {
   acum_tune t{};
-  // measured on the four-blade rotor (c2 geometry, 256 observers): a 32-bin window doubles the
-  // waves per CU over 64 bins (LDS-bound occupancy) and runs 1.6x faster in both precisions
-  t.window = 32;
-  t.max_spread = 4;
-  t.max_tile = prec == ACUM_MIXED ? 128 : 64;
+  // measured on the four-blade rotor (4 x 20,000 elements, 1024 steps, 256 observers): the
+  // eval kernel's occupancy is bound by its LDS window, and a 24-bin window with 3-bin tiles ran
+  // fastest in both precisions (0.10 s mixed, 0.20 s FP64 per pass, against 0.21 s and 0.27 s
+  // with 64 and 48 bins)
+  t.window = 24;
+  t.max_spread = 3;
+  t.max_tile = 256;
   t.groups = 0;
   t.max_chunk = 1 << 20;
   t.waves = 1;
@@ -555,7 +587,7 @@ int acum_ctx_create(acum_ctx** out, int device) {
       delete c;
       return ACUM_ERR_DEVICE;
     }
-  if (hipMalloc(&c->d_counter, sizeof(int)) != hipSuccess ||
+  if (hipMalloc(&c->d_counter, 2 * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_flags, sizeof(uint32_t)) != hipSuccess) {
     delete c;
     return ACUM_ERR_DEVICE;
@@ -677,8 +709,8 @@ int acum_tonal_bins(acum_ctx* ctx, const acum_tonal_case* c, int64_t iobs, int32
   if (!rc && hipMalloc(&d, 4 * n * sizeof(int32_t)) != hipSuccess) rc = ctx_fail(ctx, ACUM_ERR_NOMEM, "hipMalloc");
   if (!rc) {
     const double ox = c->rObs[3 * iobs], oy = c->rObs[3 * iobs + 1], oz = c->rObs[3 * iobs + 2];
-    hipError_t e = launch_tonal_bins(c->precision == ACUM_MIXED, ctx->stream, sc.d_rec[c->precision], sc.nE,
-                                     sc.k, ox, oy, oz, d, d + n, d + 2 * n, d + 3 * n);
+    hipError_t e = launch_tonal_bins(ctx->stream, sc.d_pos, sc.nE, sc.k, ox, oy, oz, d, d + n,
+                                     d + 2 * n, d + 3 * n);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     int32_t* dst[4] = {ipts, npi, ndown, nup};
     for (int i = 0; i < 4 && e == hipSuccess; i++)
@@ -705,8 +737,8 @@ int acum_get_tuning(const acum_ctx* ctx, int32_t precision, acum_tune* t) {
 int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t) {
   if (!ctx || !t || (precision != ACUM_FP64 && precision != ACUM_MIXED)) return ACUM_ERR_ARG;
   const bool mixed = precision == ACUM_MIXED;
-  if (!(t->window == 32 || t->window == 64 || (mixed && (t->window == 48 || t->window == 128))))
-    return ctx_fail(ctx, ACUM_ERR_ARG, "window must be 32 or 64 (or 48, 128 for the mixed path)");
+  if (!eval_window_supported(mixed, t->window))
+    return ctx_fail(ctx, ACUM_ERR_ARG, "window must be 16, 24 or 32 (or 48, 64 for the mixed path)");
   if (t->max_spread < 1 || t->max_tile < 1 || t->max_chunk < 1 || t->groups < 0)
     return ctx_fail(ctx, ACUM_ERR_ARG, "bad tile limits");
   ctx->tune[precision] = *t;
@@ -721,11 +753,10 @@ int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* b
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const bool mixed = precision == ACUM_MIXED;
   std::vector<acum_tune> cand;
-  const int windows[4] = {32, 48, 64, 128};
-  for (int W : windows) {
-    if (!mixed && (W == 48 || W == 128)) continue;
-    for (int sp : {W / 16, W / 8, W / 4})
-      for (int mt : {64, 256}) {
+  for (int W : {16, 24, 32, 48, 64}) {
+    if (!eval_window_supported(mixed, W)) continue;
+    for (int sp : {1, 2, 3, 4, 6})
+      for (int mt : {64, 128, 256}) {
         acum_tune t = default_tune(precision);
         t.window = W;
         t.max_spread = std::max(1, sp);

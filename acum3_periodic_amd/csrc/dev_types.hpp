@@ -4,37 +4,47 @@
 
 namespace acum {
 
-// Element-time record: one (element, source step), wave-uniform in the tonal kernel.
-// The 18 input values stay float (they are parsed floats); the two observer-independent
-// products M^2 = (|V|/a0)^2 (core.cu:94) and V.n (core.cu:102) are formed in FP64 and kept
-// in the path's precision.
-struct Rec32 {
-  float x[3], ds, v[3], vd[3], n[3], nd[3], P, dP;
-  float MM, Vn;
-};  // 80 bytes
-struct Rec64 {
-  float x[3], ds, v[3], vd[3], n[3], nd[3], P, dP;
-  double MM, Vn;
-};  // 88 bytes
+// Element-time records: one (element, source step), wave-uniform in the tonal kernel, read with
+// scalar loads, element-major ([element][step]) like the reference AoS (io.cu:280-284).
+// PosRec holds the source position in FP64 (the exact parsed float) for the canonical observer
+// time and jt = j (dTau oM / dPsi), the step's emission time in bins.  AttrT holds the
+// observer-independent factors of the Farassat 1A integrand (core.cu:68-154), formed in FP64
+// from the parsed floats and stored in the path's precision R:
+//   v  = V / a0,  vd = dV/dt / a0                         (Mr, dMr of core.cu:93-95)
+//   l  = P n,     ld = dP/dt n + P dn/dt                   (L, dL/dt of core.cu:130-131)
+//   MM = |V|^2 / a0^2,  LM = P (V.n) / a0                  (M^2, L_M of core.cu:94,134)
+//   cT = rhoRef (V.n) dS a0 D1,  cL = dS D1,  D1 = 1/(4 pi a0)   (core.cu:53,123-124,138-141)
+struct PosRec {
+  double x, y, z, jt;
+};  // 32 bytes
+template <typename R>
+struct AttrT {
+  R v[3], vd[3], l[3], ld[3];
+  R MM, LM, cT, cL;
+};  // 64 bytes (float), 128 bytes (double)
 
 // Canonical FP64 observer time, in units of the observer-time bin dPsi; the CPU oracle forms
-// the same values with the same operations.
-//   c = fma(dz,dz, fma(dy,dy, dx*dx))                         |obs - src|^2
-//   still air: r = sqrt(c)
-//   otherwise: b = -2 fma(dz,vz, fma(dy,vy, dx*vx)); disc = fma(b, b, -(four_a c));
-//              t = (-b - sqrt(disc)) inv_2a, or (-b + sqrt(disc)) inv_2a if negative; r = t a0
-//   tau(j) = fma(r, r2b, j)       r2b = (oM / a0) / dPsi: (j dTau + r / a0) oM / dPsi
-// The bins of an eval are the integers b in (tau_prev, tau] (rising side of the tent) and in
-// (tau, tau_next) (falling side); bin b is observer time b dPsi, stored at b mod nT.
+// the same values with the same operations (no contraction other than the explicit fma).
+//   d = obs - src;  c = fma(dz,dz, fma(dy,dy, dx*dx))                 |obs - src|^2
+//   still air:  s = sqrt(c);  tau = fma(s, r2b, jt)                  r = s
+//   moving:     b = fma(dz,vz, fma(dy,vy, dx*vx));  s = sqrt(fma(b, b, c*na));
+//               t = (s - b) * inv_na;  tau = fma(t, KB, jt)          r = a0 t
+// with na = a0^2 - |vinf|^2 > 0, r2b = (oM / a0) / dPsi, KB = oM / dPsi, jt = j (dTau KB).
+// t is the positive root (-b' - sqrt(b'^2 - 4 a' c)) / (2 a') of a' t^2 + b' t + c = 0,
+// a' = |vinf|^2 - a0^2 < 0, b' = -2 d.vinf (acous_dist, core.cu:261-279), rewritten without
+// the factors of 2 and the sign flips; tau is
+// (j dTau + r / a0) oM / dPsi (core.cu:156).  The bins of an eval are the integers b in
+// (tau_prev, tau] (rising side of the tent) and in (tau, tau_next) (falling side); bin b is
+// observer time b dPsi, stored at b mod nT.
 struct TimeConst {
-  double a0, r2b, four_a, inv_2a;
-  double vx, vy, vz;
-  double inv_a0;
+  double a0, inv_a0, r2b, KB, SB;
+  double vx, vy, vz, na, inv_na;
   int32_t nT, still;  // still: vinf == 0
 };
 
 struct TonalCaseDev {
-  const void* rec;      // [nE][nT] records (float or double)
+  const PosRec* pos;    // [nE][nT]
+  const void* attr;     // [nE][nT] AttrT<float> or AttrT<double>
   const float* obs;     // [nObs][3]
   void* slab;           // [G][nObs][nT][2] partial sums (float or double)
   float* pT;            // outputs [nT][nObs]
@@ -43,7 +53,6 @@ struct TonalCaseDev {
   int64_t nE;
   int32_t nObs, G;
   TimeConst k;
-  double rhoRef, D1;    // D1 = 1/(4 pi a0)
 };
 
 struct Tile {

@@ -2,173 +2,152 @@
 //
 // Work decomposition (the reference launches one thread per (step, observer, element) with
 // float atomics on pT/pL/pA, core.cu:18-259, main.cu:194-236):
-//   * one wave = 64 observers (one per lane); element records are wave-uniform, so every record
-//     read from HBM serves 64 observers;
+//   * one wave = 64 observers (one per lane); element records are wave-uniform scalar loads, so
+//     every record read from HBM serves 64 observers;
 //   * a work item = (case, observer group, group of tiles); a tile is a run of elements that sit
 //     close together; for each tile the wave walks the source steps in passes of K steps and, for
 //     each element of the tile, the K steps of that pass in order -- one FP64 retarded solve per
 //     (element, step, observer) instead of the reference's three (core.cu:84,189,193);
-//   * each lane owns a column of an LDS window of W observer-time bins; tent-weighted
-//     contributions (core.cu:196-231) go there with LDS atomic adds, which are ordered because no
-//     other lane touches that column; after each pass the window is added into the item's row of
-//     a partial slab in HBM;
+//   * each lane owns a column of an LDS window of W observer-time bins; the tent-weighted
+//     contributions of an eval (core.cu:196-231) are read-modify-written into the 4-slot
+//     footprint ic-1 .. ic+2 around its top rising bin ic with predicated weights, rarer wider
+//     tents in a wave-uniform tail; no other lane touches the column, so the order is fixed;
+//   * after each pass the window is added into the item's row of a partial slab in HBM;
 //   * a reduction kernel sums the partial slabs of each (observer, bin) in fixed order with
 //     FP64 Kahan compensation and writes pT, pL and pA = pT + pL.
 // Results are deterministic: no operation depends on scheduling order.
 //
 // Precision: MIXED computes observer time, bin index and the bin time deltas in FP64 and the
-// geometry, Doppler factor, integrands and weights in FP32 (FP32 window and slab); PLAIN does
-// everything in FP64.  The FP64 observer-time/bin code is the canonical form the CPU oracle
-// uses, compiled without contraction, so bin indices agree bit for bit.
+// geometry, Doppler factor, integrands and tent weights in FP32 (FP32 window and slab); PLAIN
+// does everything in FP64.  The FP64 observer-time/bin code is the canonical form the CPU
+// oracle uses, compiled without contraction, so bin indices agree bit for bit.
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include "dev_types.hpp"
+#include "fp64_math.hpp"
 
 namespace acum {
 
-// ---------------------------------------------------------------------------------------------
-// Canonical FP64 observer time (acous_dist core.cu:261-279, ttemp core.cu:156) in bin units; the
-// CPU oracle evaluates the same operations in the same order.
-__device__ __forceinline__ double retarded_r(const TimeConst& k, double dx, double dy, double dz) {
-  const double c = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-  if (k.still) return sqrt(c);
-  const double b = -2.0 * __builtin_fma(dz, k.vz, __builtin_fma(dy, k.vy, dx * k.vx));
-  const double disc = __builtin_fma(b, b, -(k.four_a * c));
-  const double sq = sqrt(disc);
-  double t = (-b - sq) * k.inv_2a;
-  if (t < 0.0) t = (-b + sq) * k.inv_2a;
-  return t * k.a0;
-}
-
-__device__ __forceinline__ double obs_tau(const TimeConst& k, int j, double r) {
-  return __builtin_fma(r, k.r2b, (double)j);
-}
-
-__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
-
-// Tent of one eval (findpts/bsearch core.cu:290-355 and the sweeps core.cu:196-231, in bin
-// units).  The top rising bin is ic = floor(tau); the rising side holds nd = ic - floor(tau_prev)
-// bins ic, ic-1, ... with weights (b - tau_prev) / (tau - tau_prev); the falling side holds
-// nu = ceil(tau_next) - 1 - ic bins ic+1, ic+2, ... with weights (tau_next - b) / (tau_next - tau).
-struct Tent {
-  double fc, tp, tn;  // floor(tau), tau_prev, tau_next
-  int ic, nd, nu;
-};
-
-__device__ __forceinline__ bool tent_of(double tp, double tc, double tn, Tent* T) {
-  const double fc = floor(tc), fp = floor(tp), cn = ceil(tn);
-  const bool ok = fabs(tc) < 1.0e9 && fabs(tp) < 1.0e9 && fabs(tn) < 1.0e9;  // finite, in range
-  T->fc = fc;
-  T->tp = tp;
-  T->tn = tn;
-  T->ic = ok ? (int)fc : 0;
-  const int nd = ok ? (int)(fc - fp) : 0, nu = ok ? (int)(cn - fc) - 1 : 0;
-  T->nd = nd > 0 ? nd : 0;
-  T->nu = nu > 0 ? nu : 0;
-  return ok;
-}
-
-// ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ double rcpT(double x) { return 1.0 / x; }
-
-// Records are read through global-address-space pointers: with a wave-uniform address each
-// read is one vector memory request serving all 64 lanes, counted by vmcnt, so waiting for a
-// record never waits for the LDS atomics of the bins (which count on lgkmcnt).
+// Records are read through global-address-space pointers with wave-uniform addresses.
 template <typename T>
 using cptr = const __attribute__((address_space(1))) T*;
 
-// Loads one record as whole dwords.
-template <typename Rec>
-__device__ __forceinline__ Rec load_rec(cptr<Rec> p) {
-  static_assert(sizeof(Rec) % 4 == 0, "record must be whole dwords");
-  constexpr int n = (int)(sizeof(Rec) / 4);
+template <typename T>
+__device__ __forceinline__ T load_words(cptr<T> p) {
+  static_assert(sizeof(T) % 4 == 0, "record must be whole dwords");
+  constexpr int n = (int)(sizeof(T) / 4);
   cptr<uint32_t> w = (cptr<uint32_t>)p;
   uint32_t t[n];
 #pragma unroll
   for (int i = 0; i < n; i++) t[i] = w[i];
-  Rec r;
-  __builtin_memcpy(&r, t, sizeof(Rec));
+  T r;
+  __builtin_memcpy(&r, t, sizeof(T));
   return r;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Canonical FP64 observer time in bins (acous_dist core.cu:261-279, ttemp core.cu:156); the CPU
+// oracle evaluates the same operations in the same order.  `delay` returns r (still air, metres)
+// or t = r / a0 (moving medium, seconds).
+template <bool MOVING>
+__device__ __forceinline__ double obs_time(const TimeConst& k, double jt, double dx, double dy,
+                                           double dz, double* delay) {
+  const double c = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+  if (!MOVING) {
+    const double s = sqrt_pos(c);
+    *delay = s;
+    return __builtin_fma(s, k.r2b, jt);
+  } else {
+    const double b = __builtin_fma(dz, k.vz, __builtin_fma(dy, k.vy, dx * k.vx));
+    const double s = sqrt_pos(__builtin_fma(b, b, c * k.na));
+    const double t = (s - b) * k.inv_na;
+    *delay = t;
+    return __builtin_fma(t, k.KB, jt);
+  }
+}
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcpT(double x) { return rcp_nr(x); }
 
 template <bool MIXED>
 struct Prec {
   using R = double;
   using R2 = double2;
-  using Rec = Rec64;
 };
 template <>
 struct Prec<true> {
   using R = float;
   using R2 = float2;
-  using Rec = Rec32;
 };
 
-// Farassat 1A integrand of the impermeable surface (core.cu:68-154): rho = 1, U = V, the dotted
-// normal velocities forced to zero (core.cu:117-118), so
+// Farassat 1A integrand of the impermeable surface (core.cu:68-154): rho = 1, U = V and the
+// dotted normal velocities forced to zero (core.cu:117-118), so
 //   pThick = rhoRef Vn dS a0 (dMr T2 + a0 (Mr - M^2) T4)
-//   pLoad  = dS (dLr T1 + Lr dMr T2 + a0 (Lr - Lm) T3 + a0 Lr (Mr - M^2) T4)
-// with T1 = 1/(4 pi a0 r (1-Mr)^2), T2 = T1/(1-Mr), T3 = T1/r, T4 = T3/(1-Mr).
-template <typename R, typename Rec>
-__device__ __forceinline__ void integrand(const Rec& q, R ox, R oy, R oz, R r, R inv_a0, R a0,
-                                          R vx, R vy, R vz, R rhoRef, R D1, R* pth, R* plo,
-                                          bool* bad) {
-  const R rt = r * inv_a0;
-  const R rvx = fmaT(-vx, rt, ox - (R)q.x[0]);
-  const R rvy = fmaT(-vy, rt, oy - (R)q.x[1]);
-  const R rvz = fmaT(-vz, rt, oz - (R)q.x[2]);
+//   pLoad  = dS (dLr T1 + Lr dMr T2 + a0 (Lr - L_M) T3 + a0 Lr (Mr - M^2) T4)
+// with T1 = D1/(r (1-Mr)^2), T2 = T1/(1-Mr), T3 = T1/r, T4 = T3/(1-Mr).  With q = 1/(1-Mr),
+// T = q^2 / r and A = a0 / r this is
+//   pThick = cT T q (dMr + A (Mr - M^2))
+//   pLoad  = cL T (dLr + Lr q dMr + A ((Lr - L_M) + Lr q (Mr - M^2)))
+// (cT, cL carry the observer-independent factors, see AttrT).  d is obs - src; `del` is r in
+// still air and t = r / a0 in a moving medium, where r_vec = d - vinf t (core.cu:86).
+template <typename R, bool MOVING>
+__device__ __forceinline__ void integrand(const AttrT<R>& q, R dx, R dy, R dz, R del, R a0,
+                                          R vx, R vy, R vz, R* pth, R* plo, bool* bad) {
+  R r, rx, ry, rz;
+  if (MOVING) {
+    r = a0 * del;
+    rx = fmaT(-vx, del, dx);
+    ry = fmaT(-vy, del, dy);
+    rz = fmaT(-vz, del, dz);
+  } else {
+    r = del;
+    rx = dx;
+    ry = dy;
+    rz = dz;
+  }
+  const R dV = fmaT(q.v[2], rz, fmaT(q.v[1], ry, q.v[0] * rx));
+  const R dVd = fmaT(q.vd[2], rz, fmaT(q.vd[1], ry, q.vd[0] * rx));
+  const R dL = fmaT(q.l[2], rz, fmaT(q.l[1], ry, q.l[0] * rx));
+  const R dLd = fmaT(q.ld[2], rz, fmaT(q.ld[1], ry, q.ld[0] * rx));
   const R ir = rcpT(r);
-  const R hx = rvx * ir, hy = rvy * ir, hz = rvz * ir;
-  const R Vr = fmaT((R)q.v[0], hx, fmaT((R)q.v[1], hy, (R)q.v[2] * hz));
-  const R dVr = fmaT((R)q.vd[0], hx, fmaT((R)q.vd[1], hy, (R)q.vd[2] * hz));
-  const R Nr = fmaT((R)q.n[0], hx, fmaT((R)q.n[1], hy, (R)q.n[2] * hz));
-  const R dNr = fmaT((R)q.nd[0], hx, fmaT((R)q.nd[1], hy, (R)q.nd[2] * hz));
-  const R Mr = Vr * inv_a0;
-  const R dMr = dVr * inv_a0;
+  const R Mr = dV * ir, Mdr = dVd * ir, Lr = dL * ir, Ldr = dLd * ir;
   const R om = (R)1 - Mr;
-  const R iom = rcpT(om);
-  const R T1 = D1 * ir * iom * iom;
-  const R T2 = T1 * iom;
-  const R T3 = T1 * ir;
-  const R T4 = T3 * iom;
-  const R MmM = Mr - (R)q.MM;
-  const R ds = (R)q.ds, P = (R)q.P, dP = (R)q.dP, Vn = (R)q.Vn;
-  *pth = rhoRef * Vn * ds * a0 * fmaT(dMr, T2, a0 * MmM * T4);
-  const R Lr = P * Nr;
-  const R dLr = fmaT(dP, Nr, P * dNr);
-  const R Lm = P * Vn * inv_a0;
-  R s = dLr * T1;
-  s = fmaT(Lr * dMr, T2, s);
-  s = fmaT(a0 * (Lr - Lm), T3, s);
-  s = fmaT(a0 * Lr * MmM, T4, s);
-  *plo = ds * s;
+  const R qq = rcpT(om);
+  const R T = ir * qq * qq;
+  const R mm = Mr - q.MM;
+  const R A = a0 * ir;
+  *pth = (q.cT * T) * (qq * fmaT(A, mm, Mdr));
+  const R Lq = Lr * qq;
+  R s = fmaT(A, fmaT(Lq, mm, Lr - q.LM), Ldr);
+  s = fmaT(Lq, Mdr, s);
+  *plo = (q.cL * T) * s;
   *bad = !(r > (R)0) || !(om > (R)0);
-}
-
-template <typename Rec>
-__device__ __forceinline__ double tau_at(const TimeConst& k, const Rec& q, int j, double ox,
-                                         double oy, double oz, double* r_out) {
-  const double r = retarded_r(k, ox - (double)q.x[0], oy - (double)q.x[1], oz - (double)q.x[2]);
-  *r_out = r;
-  return obs_tau(k, j, r);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Eval + binning.  One wave per workgroup, persistent over a queue of items.
-template <bool MIXED, int W>
+//
+// Per eval (element e, step k, observer = lane) with tau_prev, tau, tau_next the observer times
+// of steps k-1, k, k+1 (wrapped by -nT / +nT at the ends of the period, core.cu:173-194):
+//   ic = floor(tau); nd = ic - floor(tau_prev) rising bins ic, ic-1, ... with weights
+//   (b - tau_prev)/(tau - tau_prev) = 1 - (fr + i)/(tau - tau_prev), fr = tau - ic;
+//   nu = ceil(tau_next) - 1 - ic falling bins ic+1, ic+2, ... with weights
+//   (tau_next - b)/(tau_next - tau) = 1 - (i + 1 - fr)/(tau_next - tau).
+// Bin counts and fr come from the FP64 observer times; the weights are formed in R.
+template <bool MIXED, bool MOVING, int W>
 __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
                                                         const Tile* __restrict__ tiles,
                                                         const Item* __restrict__ items, int nItems,
                                                         int* __restrict__ counter,
                                                         uint32_t* __restrict__ flags_out) {
   using R = typename Prec<MIXED>::R;
-  using Rec = typename Prec<MIXED>::Rec;
   using R2 = typename Prec<MIXED>::R2;
-  // win[slot][lane] = (thickness, loading) partial of one observer-time bin; each lane owns its
-  // column, so plain read-modify-writes are race free and keep a fixed order
+  using Attr = AttrT<R>;
+  // win[slot][lane] = (thickness, loading) partial of one observer-time bin
   __shared__ R2 win[W * 64];
   const int lane = threadIdx.x;
   for (int s = 0; s < W; s++) win[s * 64 + lane] = R2{(R)0, (R)0};
@@ -186,70 +165,91 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
     const bool active = o < nObs;
     const int oc = active ? o : nObs - 1;
     const double ox = Cp->obs[3 * oc], oy = Cp->obs[3 * oc + 1], oz = Cp->obs[3 * oc + 2];
-    const R oxr = (R)ox, oyr = (R)oy, ozr = (R)oz;
     R* slab = (R*)Cp->slab + ((int64_t)item.group * nObs + oc) * (int64_t)nT * 2;
-    cptr<Rec> rec = (cptr<Rec>)(uintptr_t)Cp->rec;
-    const R inv_a0 = (R)k.inv_a0, a0 = (R)k.a0, vx = (R)k.vx, vy = (R)k.vy, vz = (R)k.vz;
-    const R rhoRef = (R)Cp->rhoRef, D1 = (R)Cp->D1;
+    cptr<PosRec> pos = (cptr<PosRec>)(uintptr_t)Cp->pos;
+    cptr<Attr> attr = (cptr<Attr>)(uintptr_t)Cp->attr;
+    const R a0 = (R)k.a0, vx = (R)k.vx, vy = (R)k.vy, vz = (R)k.vz;
     const double dnT = (double)nT;
     for (int ti = item.tile0; ti < item.tile0 + item.ntile; ti++) {
       const Tile tl = tiles[ti];
       for (int k0 = 0; k0 < nT; k0 += tl.K) {
         const int kend = min(k0 + tl.K, nT);
         const int km0 = (k0 > 0) ? k0 - 1 : nT - 1;
+        const int kn0 = (k0 + 1 < nT) ? k0 + 1 : 0;
         int base;
         {
-          double r;
-          double tb = tau_at(k, load_rec<Rec>(rec + tl.e0 * nT + km0), km0, ox, oy, oz, &r);
+          const PosRec p = load_words<PosRec>(pos + tl.e0 * nT + km0);
+          double d;
+          double tb = obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &d);
           if (k0 == 0) tb = tb - dnT;
-          base = (int)floor(tb) - tl.spread - 3;
+          base = (int)floor(tb) - tl.spread - 1;
         }
-        int smin = W, smax = -1;
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
-          cptr<Rec> re = rec + e * nT;
-          double r_cur, r_tmp;
-          double t_prev = tau_at(k, load_rec<Rec>(re + km0), km0, ox, oy, oz, &r_tmp);
-          if (k0 == 0) t_prev = t_prev - dnT;
-          // two-deep record pipeline: qc = step kk (integrand), qn = step kk+1 (retarded solve),
-          // qnn = step kk+2 in flight
-          Rec qc = load_rec<Rec>(re + k0);
-          const int kn0 = (k0 + 1 < nT) ? k0 + 1 : 0;
-          Rec qn = load_rec<Rec>(re + kn0);
-          double t_cur = tau_at(k, qc, k0, ox, oy, oz, &r_cur);
+          cptr<PosRec> pe = pos + e * nT;
+          cptr<Attr> ae = attr + e * nT;
+          // state at the start of the pass: tau(k0-1), tau(k0)
+          double t_prev, t_cur, dl;
+          {
+            const PosRec p = load_words<PosRec>(pe + km0);
+            t_prev = obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &dl);
+            if (k0 == 0) t_prev = t_prev - dnT;
+          }
+          R dxc, dyc, dzc, delc;
+          {
+            const PosRec p = load_words<PosRec>(pe + k0);
+            const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
+            t_cur = obs_time<MOVING>(k, p.jt, dx, dy, dz, &dl);
+            dxc = (R)dx;
+            dyc = (R)dy;
+            dzc = (R)dz;
+            delc = (R)dl;
+          }
+          if (!(fabs(t_prev) < 1.0e9) || !(fabs(t_cur) < 1.0e9)) flags |= kFlagNumeric;
+          int ic_p = (int)floor(t_prev);
+          double fl_c = floor(t_cur);
+          int ic = (int)fl_c;
+          // record pipeline: positions one step ahead of the solve, attributes one step ahead
+          // of the integrand, each with one load in flight
+          PosRec pn = load_words<PosRec>(pe + kn0);
+          Attr ac = load_words<Attr>(ae + k0);
           for (int kk = k0; kk < kend; kk++) {
             const int kn = (kk + 1 < nT) ? kk + 1 : 0;
             const int knn = (kn + 1 < nT) ? kn + 1 : 0;
-            const Rec qnn = load_rec<Rec>(re + knn);
-            // the rising side of this eval's tent is known before the solve: start the window
-            // reads of its footprint now
-            const double fc = floor(t_cur);
-            int sb = (int)fc - 1 - base;
+            const PosRec pnn = load_words<PosRec>(pe + knn);
+            const Attr an = load_words<Attr>(ae + kn);
+            // the footprint is known before the solve: start its window reads now
+            int sb = ic - 1 - base;
             const bool inrange = sb >= 0 && sb + 3 < W;
-            sb = inrange ? sb : (sb < 0 ? 0 : W - 4);
-            R2* p = &win[sb * 64 + lane];
-            R2 v0 = p[0], v1 = p[64], v2 = p[128], v3 = p[192];
+            sb = inrange ? sb : 0;
+            R2* wp = &win[sb * 64 + lane];
+            R2 v0 = wp[0], v1 = wp[64], v2 = wp[128], v3 = wp[192];
+            // observer time of the next step
+            const double dxn = ox - pn.x, dyn = oy - pn.y, dzn = oz - pn.z;
+            double dln;
+            double t_next = obs_time<MOVING>(k, pn.jt, dxn, dyn, dzn, &dln);
+            if (kk == nT - 1) t_next = t_next + dnT;
+            // integrand of this step
             R pth, plo;
             bool bad;
-            integrand<R, Rec>(qc, oxr, oyr, ozr, (R)r_cur, inv_a0, a0, vx, vy, vz, rhoRef, D1,
-                              &pth, &plo, &bad);
-            const double r_next =
-                retarded_r(k, ox - (double)qn.x[0], oy - (double)qn.x[1], oz - (double)qn.x[2]);
-            double t_next = obs_tau(k, kn, r_next);
-            if (kk == nT - 1) t_next = t_next + dnT;
-            Tent T;
-            const bool ok = tent_of(t_prev, t_cur, t_next, &T) && !bad;
+            integrand<R, MOVING>(ac, dxc, dyc, dzc, delc, a0, vx, vy, vz, &pth, &plo, &bad);
+            // tent
+            const double fl_n = floor(t_next);
+            const int ic_n = (int)fl_n;
+            const int nd = ic - ic_p;
+            const int nu = ic_n - ic - (t_next == fl_n ? 1 : 0);
+            const bool ok = !bad && fabs(t_next) < 1.0e9;
             if (!ok) flags |= kFlagNumeric;
             const R pthv = ok ? pth : (R)0, plov = ok ? plo : (R)0;
-            const R idm = rcpT((R)(t_cur - t_prev));
-            const R idp = rcpT((R)(t_next - t_cur));
-            // footprint slots ic-1 .. ic+2: rising bins ic-1 (nd >= 2), ic (nd >= 1);
-            // falling bins ic+1 (nu >= 1), ic+2 (nu >= 2)
-            const R w0 = T.nd >= 2 ? (R)((fc - 1.0) - t_prev) * idm : (R)0;
-            const R w1 = T.nd >= 1 ? (R)(fc - t_prev) * idm : (R)0;
-            const R w2 = T.nu >= 1 ? (R)(t_next - (fc + 1.0)) * idp : (R)0;
-            const R w3 = T.nu >= 2 ? (R)(t_next - (fc + 2.0)) * idp : (R)0;
-            const bool any_on = (T.nd | T.nu) != 0;
-            if (any_on && !inrange) flags |= kFlagWindow;
+            const R fr = (R)(t_cur - fl_c);
+            const R idp = rcpT((R)(t_cur - t_prev));
+            const R idn = rcpT((R)(t_next - t_cur));
+            const R wr0 = fmaT(-fr, idp, (R)1);       // bin ic
+            const R wf0 = fmaT(fr - (R)1, idn, (R)1);  // bin ic+1
+            const R w1 = nd >= 1 ? wr0 : (R)0;
+            const R w0 = nd >= 2 ? wr0 - idp : (R)0;   // bin ic-1
+            const R w2 = nu >= 1 ? wf0 : (R)0;
+            const R w3 = nu >= 2 ? wf0 - idn : (R)0;   // bin ic+2
+            if (!inrange && (nd | nu) != 0) flags |= kFlagWindow;
             v0.x = fmaT(pthv, w0, v0.x);
             v0.y = fmaT(plov, w0, v0.y);
             v1.x = fmaT(pthv, w1, v1.x);
@@ -258,19 +258,15 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
             v2.y = fmaT(plov, w2, v2.y);
             v3.x = fmaT(pthv, w3, v3.x);
             v3.y = fmaT(plov, w3, v3.y);
-            p[0] = v0;
-            p[64] = v1;
-            p[128] = v2;
-            p[192] = v3;
-            if (any_on) {
-              smin = min(smin, sb);
-              smax = max(smax, sb + 3);
-            }
-            const bool tail = T.nd > 2 || T.nu > 2;
+            wp[0] = v0;
+            wp[64] = v1;
+            wp[128] = v2;
+            wp[192] = v3;
+            const bool tail = nd > 2 || nu > 2;
             if (wave_any(tail)) {  // tents wider than two bins a side
               if (tail) {
-                if (T.nd > W || T.nu > W) flags |= kFlagSweep;
-                const int nd = min(T.nd, W), nu = min(T.nu, W);
+                if (nd > W || nu > W) flags |= kFlagSweep;
+                const int nd_ = min(nd, W), nu_ = min(nu, W);
                 auto put = [&](int b, R w) {
                   const int slot = b - base;
                   if (slot < 0 || slot >= W) {
@@ -281,30 +277,37 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
                   v.x = fmaT(pthv, w, v.x);
                   v.y = fmaT(plov, w, v.y);
                   win[slot * 64 + lane] = v;
-                  smin = min(smin, slot);
-                  smax = max(smax, slot);
                 };
-                for (int j = 2; j < nd; j++) put(T.ic - j, (R)((fc - (double)j) - t_prev) * idm);
-                for (int j = 2; j < nu; j++) put(T.ic + 1 + j, (R)(t_next - (fc + (double)(1 + j))) * idp);
+                for (int j = 2; j < nd_; j++) put(ic - j, fmaT(-(fr + (R)j), idp, (R)1));
+                for (int j = 2; j < nu_; j++) put(ic + 1 + j, fmaT(fr - (R)(1 + j), idn, (R)1));
               }
             }
             t_prev = t_cur;
             t_cur = t_next;
-            r_cur = r_next;
-            qc = qn;
-            qn = qnn;
+            ic_p = ic;
+            ic = ic_n;
+            fl_c = fl_n;
+            dxc = (R)dxn;
+            dyc = (R)dyn;
+            dzc = (R)dzn;
+            delc = (R)dln;
+            pn = pnn;
+            ac = an;
           }
         }
-        // add the window into this item's slab row and clear it
-        for (int s = smin; s <= smax; s++) {
+        // add the window into this item's slab row and clear it (untouched slots hold zeros)
+        int b = base % nT;
+        if (b < 0) b += nT;
+        for (int s = 0; s < W; s++) {
           const R2 v = win[s * 64 + lane];
-          win[s * 64 + lane] = R2{(R)0, (R)0};
-          if (active) {
-            int b = (base + s) % nT;
-            if (b < 0) b += nT;
-            slab[2 * b] += v.x;
-            slab[2 * b + 1] += v.y;
+          if (v.x != (R)0 || v.y != (R)0) {
+            win[s * 64 + lane] = R2{(R)0, (R)0};
+            if (active) {
+              slab[2 * b] += v.x;
+              slab[2 * b + 1] += v.y;
+            }
           }
+          b = (b + 1 == nT) ? 0 : b + 1;
         }
       }
     }
@@ -356,33 +359,50 @@ __global__ __launch_bounds__(256) void tonal_reduce_kernel(const TonalCaseDev* _
   }
 }
 
-// Reference AoS (X[4], N[6], V[6], PR[2] per element-step, surfaces concatenated) -> records.
-template <typename Rec, typename H>
-__global__ void tonal_prep_kernel(const float* __restrict__ X, const float* __restrict__ N,
+// Reference AoS (X[4], N[6], V[6], PR[2] per element-step, surfaces concatenated, element-major
+// with nT steps each) -> position and attribute records; the products are formed in FP64.
+__global__ void tonal_pos_kernel(const float* __restrict__ X, int64_t n, int nT, double SB,
+                                 PosRec* __restrict__ pos) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const float* x = X + 4 * q;
+    PosRec o;
+    o.x = x[0];
+    o.y = x[1];
+    o.z = x[2];
+    o.jt = (double)(int)(q % nT) * SB;
+    pos[q] = o;
+  }
+}
+
+template <typename R>
+__global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __restrict__ N,
                                   const float* __restrict__ V, const float* __restrict__ PR,
-                                  int64_t n, double inv_a0, Rec* __restrict__ rec) {
+                                  int64_t n, double a0, double rhoRef, AttrT<R>* __restrict__ attr) {
+  const double inv_a0 = 1.0 / a0;
+  const double D1 = 1.0 / (4.0 * M_PI * a0);
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
     const float* x = X + 4 * q;
     const float* nn = N + 6 * q;
     const float* v = V + 6 * q;
-    const float* p = PR + 2 * q;
-    Rec o;
+    const double P = PR[2 * q], dP = PR[2 * q + 1], ds = x[3];
+    AttrT<R> o;
+    double Vn = 0.0, VV = 0.0;
     for (int i = 0; i < 3; i++) {
-      o.x[i] = x[i];
-      o.v[i] = v[i];
-      o.vd[i] = v[3 + i];
-      o.n[i] = nn[i];
-      o.nd[i] = nn[3 + i];
+      const double Vi = v[i], Vdi = v[3 + i], ni = nn[i], ndi = nn[3 + i];
+      o.v[i] = (R)(Vi * inv_a0);
+      o.vd[i] = (R)(Vdi * inv_a0);
+      o.l[i] = (R)(P * ni);
+      o.ld[i] = (R)(dP * ni + P * ndi);
+      Vn += Vi * ni;
+      VV += Vi * Vi;
     }
-    o.ds = x[3];
-    o.P = p[0];
-    o.dP = p[1];
-    const double vv = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
-    const double M = sqrt(vv) * inv_a0;
-    o.MM = (H)(M * M);
-    o.Vn = (H)((double)v[0] * nn[0] + (double)v[1] * nn[1] + (double)v[2] * nn[2]);
-    rec[q] = o;
+    o.MM = (R)(VV * inv_a0 * inv_a0);
+    o.LM = (R)(P * Vn * inv_a0);
+    o.cT = (R)(rhoRef * Vn * ds * a0 * D1);
+    o.cL = (R)(ds * D1);
+    attr[q] = o;
   }
 }
 
@@ -394,38 +414,47 @@ __global__ void fill_zero_kernel(T* __restrict__ p, int64_t n) {
 }
 
 __global__ void reset_counter_kernel(int* c, uint32_t* flags) {
-  *c = 0;
+  c[0] = 0;  // still-air item queue
+  c[1] = 0;  // moving-medium item queue
   *flags = 0u;
 }
 
-// Tent of every (element, step) for one observer, through the same tent code: the top rising
-// bin ic (stored as ic mod nT, with its period index floor(ic / nT)) and the two side lengths.
-template <typename Rec>
-__global__ void tonal_bins_kernel(const Rec* __restrict__ rec, int64_t nE, TimeConst k, double ox,
-                                  double oy, double oz, int32_t* ipts, int32_t* npi, int32_t* nd,
-                                  int32_t* nu) {
+// Tent of every (element, step) for one observer, from the same canonical observer time: the
+// top rising bin ic (stored as ic mod nT, with its period index floor(ic / nT)) and the two side
+// lengths.
+template <bool MOVING>
+__global__ void tonal_bins_kernel(const PosRec* __restrict__ pos, int64_t nE, TimeConst k,
+                                  double ox, double oy, double oz, int32_t* ipts, int32_t* npi,
+                                  int32_t* nd, int32_t* nu) {
   const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (e >= nE) return;
   const int nT = k.nT;
-  const Rec* re = rec + e * nT;
-  double r;
-  double t_prev = tau_at(k, re[nT - 1], nT - 1, ox, oy, oz, &r) - (double)nT;
-  double t_cur = tau_at(k, re[0], 0, ox, oy, oz, &r);
+  const PosRec* pe = pos + e * nT;
+  auto tau = [&](int j) {
+    const PosRec p = pe[j];
+    double d;
+    return obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &d);
+  };
+  double t_prev = tau(nT - 1) - (double)nT;
+  double t_cur = tau(0);
   for (int kk = 0; kk < nT; kk++) {
-    double t_next = (kk + 1 < nT) ? tau_at(k, re[kk + 1], kk + 1, ox, oy, oz, &r)
-                                  : tau_at(k, re[0], 0, ox, oy, oz, &r) + (double)nT;
+    const double t_next = (kk + 1 < nT) ? tau(kk + 1) : tau(0) + (double)nT;
     const int64_t idx = e * nT + kk;
-    Tent T;
-    if (tent_of(t_prev, t_cur, t_next, &T)) {
-      int per = T.ic / nT, ph = T.ic % nT;
+    const bool ok = fabs(t_prev) < 1.0e9 && fabs(t_cur) < 1.0e9 && fabs(t_next) < 1.0e9;
+    if (ok) {
+      const double fc = floor(t_cur);
+      const int ic = (int)fc;
+      const int d = ic - (int)floor(t_prev);
+      const int u = (int)ceil(t_next) - 1 - ic;
+      int per = ic / nT, ph = ic % nT;
       if (ph < 0) {
         ph += nT;
         per -= 1;
       }
       ipts[idx] = ph;
       npi[idx] = per;
-      nd[idx] = T.nd;
-      nu[idx] = T.nu;
+      nd[idx] = d > 0 ? d : 0;
+      nu[idx] = u > 0 ? u : 0;
     } else {
       ipts[idx] = -1;
       npi[idx] = 0;
@@ -439,39 +468,46 @@ __global__ void tonal_bins_kernel(const Rec* __restrict__ rec, int64_t nE, TimeC
 
 // ---------------------------------------------------------------------------------------------
 // Host-side launchers (called from the C-ABI layer).
-hipError_t launch_tonal_eval(bool mixed, int W, int grid, hipStream_t s, const TonalCaseDev* cases,
-                             const Tile* tiles, const Item* items, int nItems, int* counter,
-                             uint32_t* flags) {
+#define ACUM_EVAL_VARIANTS(X) \
+  X(true, 16) X(true, 24) X(true, 32) X(true, 48) X(true, 64) X(false, 16) X(false, 24) X(false, 32)
+
+hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
+                             const TonalCaseDev* cases, const Tile* tiles, const Item* items,
+                             int nItems, int* counter, uint32_t* flags) {
   dim3 b(64);
-#define ACUM_EVAL(M, WW)                                                                     \
-  if (mixed == M && W == WW) {                                                               \
-    hipLaunchKernelGGL((tonal_eval_kernel<M, WW>), dim3(grid), b, 0, s, cases, tiles, items, \
-                       nItems, counter, flags);                                              \
-    return hipGetLastError();                                                                \
+#define ACUM_EVAL(M, WW)                                                                       \
+  if (mixed == M && W == WW) {                                                                 \
+    if (moving)                                                                                \
+      hipLaunchKernelGGL((tonal_eval_kernel<M, true, WW>), dim3(grid), b, 0, s, cases, tiles,  \
+                         items, nItems, counter, flags);                                       \
+    else                                                                                       \
+      hipLaunchKernelGGL((tonal_eval_kernel<M, false, WW>), dim3(grid), b, 0, s, cases, tiles, \
+                         items, nItems, counter, flags);                                       \
+    return hipGetLastError();                                                                  \
   }
-  ACUM_EVAL(true, 32)
-  ACUM_EVAL(true, 48)
-  ACUM_EVAL(true, 64)
-  ACUM_EVAL(true, 128)
-  ACUM_EVAL(false, 32)
-  ACUM_EVAL(false, 64)
+  ACUM_EVAL_VARIANTS(ACUM_EVAL)
 #undef ACUM_EVAL
   return hipErrorInvalidValue;
 }
 
-hipError_t eval_occupancy(bool mixed, int W, int* blocks_per_cu) {
-#define ACUM_OCC(M, WW)                                                                      \
-  if (mixed == M && W == WW)                                                                 \
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                       \
-                                                        tonal_eval_kernel<M, WW>, 64, 0);
-  ACUM_OCC(true, 32)
-  ACUM_OCC(true, 48)
-  ACUM_OCC(true, 64)
-  ACUM_OCC(true, 128)
-  ACUM_OCC(false, 32)
-  ACUM_OCC(false, 64)
+hipError_t eval_occupancy(bool mixed, bool moving, int W, int* blocks_per_cu) {
+#define ACUM_OCC(M, WW)                                                                        \
+  if (mixed == M && W == WW)                                                                   \
+    return moving ? hipOccupancyMaxActiveBlocksPerMultiprocessor(                              \
+                        blocks_per_cu, tonal_eval_kernel<M, true, WW>, 64, 0)                  \
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(                              \
+                        blocks_per_cu, tonal_eval_kernel<M, false, WW>, 64, 0);
+  ACUM_EVAL_VARIANTS(ACUM_OCC)
 #undef ACUM_OCC
   return hipErrorInvalidValue;
+}
+
+bool eval_window_supported(bool mixed, int W) {
+#define ACUM_HAS(M, WW) \
+  if (mixed == M && W == WW) return true;
+  ACUM_EVAL_VARIANTS(ACUM_HAS)
+#undef ACUM_HAS
+  return false;
 }
 
 hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* cases, int icase,
@@ -484,15 +520,22 @@ hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* ca
   return hipGetLastError();
 }
 
-hipError_t launch_tonal_prep(bool mixed, hipStream_t s, const float* X, const float* N,
-                             const float* V, const float* PR, int64_t n, double inv_a0, void* rec) {
+hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, int nT, double SB, void* pos) {
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(tonal_pos_kernel, dim3(grid), dim3(256), 0, s, X, n, nT, SB, (PosRec*)pos);
+  return hipGetLastError();
+}
+
+hipError_t launch_tonal_attr(bool mixed, hipStream_t s, const float* X, const float* N,
+                             const float* V, const float* PR, int64_t n, double a0, double rhoRef,
+                             void* attr) {
   const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
   if (mixed)
-    hipLaunchKernelGGL((tonal_prep_kernel<Rec32, float>), dim3(grid), dim3(256), 0, s, X, N, V, PR, n,
-                       inv_a0, (Rec32*)rec);
+    hipLaunchKernelGGL(tonal_attr_kernel<float>, dim3(grid), dim3(256), 0, s, X, N, V, PR, n, a0,
+                       rhoRef, (AttrT<float>*)attr);
   else
-    hipLaunchKernelGGL((tonal_prep_kernel<Rec64, double>), dim3(grid), dim3(256), 0, s, X, N, V, PR, n,
-                       inv_a0, (Rec64*)rec);
+    hipLaunchKernelGGL(tonal_attr_kernel<double>, dim3(grid), dim3(256), 0, s, X, N, V, PR, n, a0,
+                       rhoRef, (AttrT<double>*)attr);
   return hipGetLastError();
 }
 
@@ -512,15 +555,15 @@ hipError_t launch_reset(hipStream_t s, int* counter, uint32_t* flags) {
   return hipGetLastError();
 }
 
-hipError_t launch_tonal_bins(bool mixed, hipStream_t s, const void* rec, int64_t nE, const TimeConst& k,
+hipError_t launch_tonal_bins(hipStream_t s, const void* pos, int64_t nE, const TimeConst& k,
                              double ox, double oy, double oz, int32_t* ipts, int32_t* npi,
                              int32_t* nd, int32_t* nu) {
   const int grid = (int)((nE + 63) / 64);
-  if (mixed)
-    hipLaunchKernelGGL(tonal_bins_kernel<Rec32>, dim3(grid), dim3(64), 0, s, (const Rec32*)rec, nE, k,
-                       ox, oy, oz, ipts, npi, nd, nu);
+  if (k.still)
+    hipLaunchKernelGGL(tonal_bins_kernel<false>, dim3(grid), dim3(64), 0, s, (const PosRec*)pos, nE,
+                       k, ox, oy, oz, ipts, npi, nd, nu);
   else
-    hipLaunchKernelGGL(tonal_bins_kernel<Rec64>, dim3(grid), dim3(64), 0, s, (const Rec64*)rec, nE,
+    hipLaunchKernelGGL(tonal_bins_kernel<true>, dim3(grid), dim3(64), 0, s, (const PosRec*)pos, nE,
                        k, ox, oy, oz, ipts, npi, nd, nu);
   return hipGetLastError();
 }

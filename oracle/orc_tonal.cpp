@@ -14,10 +14,13 @@
 // kernels (same operations, same order, explicit fma, no other contraction), so bin indices
 // agree exactly.  Time is measured in observer-time bins (units of dPsi):
 //   d = obs - src; c = fma(dz,dz, fma(dy,dy, dx*dx))                      (core.cu:267)
-//   still air: r = sqrt(c); otherwise a = |vinf|^2 - a0^2, b = -2 fma(dz,vz, fma(dy,vy, dx*vx)),
-//   disc = fma(b, b, -(4a c)), t = (-b - sqrt(disc)) / (2a), or the + root if negative, r = t a0
-//                                                                          (core.cu:261-279)
-//   tau(j) = fma(r, (oM/a0)/dPsi, j) = (j dTau + r/a0) oM / dPsi          (core.cu:156)
+//   still air: s = sqrt(c), r = s, tau(j) = fma(s, (oM/a0)/dPsi, jt)
+//   otherwise: b = fma(dz,vz, fma(dy,vy, dx*vx)), s = sqrt(fma(b, b, c na)),
+//              t = (s - b) / na, r = a0 t, tau(j) = fma(t, oM/dPsi, jt)
+//   with na = a0^2 - |vinf|^2 and jt = j (dTau oM/dPsi): tau = (j dTau + r/a0) oM / dPsi
+//   (core.cu:156).  t is the root (-b' - sqrt(b'^2 - 4a'c))/(2a') acous_dist takes
+//   (core.cu:261-279), a' = |vinf|^2 - a0^2, b' = -2 d.vinf, positive for subsonic vinf; the
+//   form above drops the factors of 2 and the sign flips.
 // findpts/bsearch (core.cu:290-355) pick the grid time at or below the shifted observer time
 // and the two sweeps (core.cu:196-231) visit the grid times in (ttem, ttemp] (weights rising
 // from 0 to 1) and in (ttemp, ttep) (weights falling from 1 to 0), wrapping periodically.  In
@@ -36,8 +39,8 @@
 namespace {
 
 struct TimeConst {
-  double a0, r2b, four_a, inv_2a;
-  double vx, vy, vz;
+  double a0, inv_a0, r2b, KB, SB;
+  double vx, vy, vz, na, inv_na;
   int64_t nT;
   bool still;  // vinf == 0
 };
@@ -45,7 +48,10 @@ struct TimeConst {
 TimeConst make_time_const(const acum_tonal_case* c) {
   TimeConst k;
   k.a0 = c->a0;
+  k.inv_a0 = 1.0 / c->a0;
   k.r2b = (c->oM / c->a0) / c->dPsi;
+  k.KB = c->oM / c->dPsi;
+  k.SB = c->dTau * k.KB;
   k.nT = c->nTime;
   k.vx = c->vInf[0];
   k.vy = c->vInf[1];
@@ -53,33 +59,31 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   double vv = k.vx * k.vx;
   vv = vv + k.vy * k.vy;
   vv = vv + k.vz * k.vz;
-  double a = vv - k.a0 * k.a0;
-  k.four_a = 4.0 * a;
-  k.inv_2a = 1.0 / (2.0 * a);
+  k.na = k.a0 * k.a0 - vv;
+  k.inv_na = 1.0 / k.na;
   k.still = (k.vx == 0.0 && k.vy == 0.0 && k.vz == 0.0);
   return k;
 }
 
-// Convected retarded distance r (acous_dist, core.cu:261-279): the smallest positive root of
-// (|vinf|^2 - a0^2) t^2 - 2 (d.vinf) t + |d|^2 = 0, times a0; r = |d| in still air.
-inline double retarded_r(const TimeConst& k, double ox, double oy, double oz,
-                         const float* src) {
+// Observer time of step j in bins and the retarded distance r (acous_dist, core.cu:261-279;
+// ttemp, core.cu:156).
+inline double obs_tau_r(const TimeConst& k, int64_t j, double ox, double oy, double oz,
+                        const float* src, double* r) {
   const double dx = ox - (double)src[0];
   const double dy = oy - (double)src[1];
   const double dz = oz - (double)src[2];
   const double c = std::fma(dz, dz, std::fma(dy, dy, dx * dx));
-  if (k.still) return std::sqrt(c);
-  const double b = -2.0 * std::fma(dz, k.vz, std::fma(dy, k.vy, dx * k.vx));
-  const double disc = std::fma(b, b, -(k.four_a * c));
-  const double sq = std::sqrt(disc);
-  double t = (-b - sq) * k.inv_2a;
-  if (t < 0.0) t = (-b + sq) * k.inv_2a;
-  return t * k.a0;
-}
-
-// Observer time of step j in bins, (j dTau + r/a0) oM / dPsi (core.cu:156).
-inline double obs_tau(const TimeConst& k, int64_t j, double r) {
-  return std::fma(r, k.r2b, (double)j);
+  const double jt = (double)j * k.SB;
+  if (k.still) {
+    const double s = std::sqrt(c);
+    *r = s;
+    return std::fma(s, k.r2b, jt);
+  }
+  const double b = std::fma(dz, k.vz, std::fma(dy, k.vy, dx * k.vx));
+  const double s = std::sqrt(std::fma(b, b, c * k.na));
+  const double t = (s - b) * k.inv_na;
+  *r = t * k.a0;
+  return std::fma(t, k.KB, jt);
 }
 
 struct Eval {
@@ -173,7 +177,7 @@ int tonal_impl(const acum_tonal_case* c, Out* pT, Out* pL, Out* pA, int nthreads
 #pragma omp parallel
   {
     std::vector<R> bT(nT), bL(nT), bA(nT);
-    std::vector<double> rr(nT);
+    std::vector<double> rr(nT), tt(nT);
 #pragma omp for schedule(dynamic, 1)
     for (int64_t o = 0; o < nObs; o++) {
       std::fill(bT.begin(), bT.end(), (R)0);
@@ -189,14 +193,14 @@ int tonal_impl(const acum_tonal_case* c, Out* pT, Out* pL, Out* pA, int nthreads
           const float* Pe = c->PR + c->PRPtr[s] + j * nT * 2;
           // One retarded solve per (element, step); the reference solves three times
           // (core.cu:84,189,193) with identical operands, hence identical values.
-          for (int64_t t = 0; t < nT; t++) rr[t] = retarded_r(k, ox, oy, oz, Xe + 4 * t);
+          for (int64_t t = 0; t < nT; t++) tt[t] = obs_tau_r(k, t, ox, oy, oz, Xe + 4 * t, &rr[t]);
           for (int64_t t = 0; t < nT && ok; t++) {
             const int64_t mm = (t > 0) ? t - 1 : nT - 1;       // core.cu:173-179
             const int64_t mp = (t < nT - 1) ? t + 1 : 0;       // core.cu:180-186
-            const double tc = obs_tau(k, t, rr[t]);
-            double tp = obs_tau(k, mm, rr[mm]);
+            const double tc = tt[t];
+            double tp = tt[mm];
             if (t == 0) tp = tp - (double)nT;  // previous step of the periodic history
-            double tn = obs_tau(k, mp, rr[mp]);
+            double tn = tt[mp];
             if (t == nT - 1) tn = tn + (double)nT;
             R pth, plo;
             integrand<R>(Xe + 4 * t, Ne + 6 * t, Ve + 6 * t, Pe + 2 * t, ox, oy, oz, rr[t], k,
@@ -245,19 +249,19 @@ int orc_tonal_bins(const acum_tonal_case* c, int64_t iobs, int32_t* ipts, int32_
   const TimeConst k = make_time_const(c);
   const int64_t nT = c->nTime;
   const double ox = c->rObs[3 * iobs], oy = c->rObs[3 * iobs + 1], oz = c->rObs[3 * iobs + 2];
-  std::vector<double> rr(nT);
+  std::vector<double> rr(nT), tt(nT);
   int64_t idx = 0;
   for (int64_t s = 0; s < c->nSurf; s++) {
     for (int64_t j = 0; j < c->E[s]; j++) {
       const float* Xe = c->X + c->XPtr[s] + j * nT * 4;
-      for (int64_t t = 0; t < nT; t++) rr[t] = retarded_r(k, ox, oy, oz, Xe + 4 * t);
+      for (int64_t t = 0; t < nT; t++) tt[t] = obs_tau_r(k, t, ox, oy, oz, Xe + 4 * t, &rr[t]);
       for (int64_t t = 0; t < nT; t++, idx++) {
         const int64_t mm = (t > 0) ? t - 1 : nT - 1;
         const int64_t mp = (t < nT - 1) ? t + 1 : 0;
-        const double tc = obs_tau(k, t, rr[t]);
-        double tp = obs_tau(k, mm, rr[mm]);
+        const double tc = tt[t];
+        double tp = tt[mm];
         if (t == 0) tp = tp - (double)nT;
-        double tn = obs_tau(k, mp, rr[mp]);
+        double tn = tt[mp];
         if (t == nT - 1) tn = tn + (double)nT;
         int64_t nd = -1, nu = -1, ic = 0;
         if (!bin_eval(k, tp, tc, tn, &nd, &nu, &ic, [](int64_t, double) {})) return ACUM_ERR_NUMERIC;
@@ -279,13 +283,15 @@ int orc_tonal_bins(const acum_tonal_case* c, int64_t iobs, int32_t* ipts, int32_
 // Observer time (bins) of one (element position, step) in the canonical FP64 form (for tests).
 double orc_obs_tau(const acum_tonal_case* c, const float* src, const float* obs, int64_t j) {
   const TimeConst k = make_time_const(c);
-  double r = retarded_r(k, obs[0], obs[1], obs[2], src);
-  return obs_tau(k, j, r);
+  double r;
+  return obs_tau_r(k, j, obs[0], obs[1], obs[2], src, &r);
 }
 
 double orc_retarded_r(const acum_tonal_case* c, const float* src, const float* obs) {
   const TimeConst k = make_time_const(c);
-  return retarded_r(k, obs[0], obs[1], obs[2], src);
+  double r;
+  obs_tau_r(k, 0, obs[0], obs[1], obs[2], src, &r);
+  return r;
 }
 
 }  // extern "C"

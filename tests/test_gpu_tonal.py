@@ -106,16 +106,32 @@ def test_batch_equals_single(ctx):
             assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("window,spread", [(32, 2), (64, 16), (128, 16)])
-def test_tile_shapes_agree(ctx, window, spread):
-    run, c = small_case(2, acum.MIXED)
+@pytest.mark.parametrize("prec,window,spread,tile", [
+    (acum.MIXED, 16, 2, 32), (acum.MIXED, 48, 6, 64), (acum.MIXED, 64, 8, 256),
+    (acum.FP64, 16, 1, 16), (acum.FP64, 32, 6, 128)])
+def test_tile_shapes_agree(ctx, prec, window, spread, tile):
+    """Window size, tile spread and tile length change only the partial-sum grouping."""
+    run, c = small_case(2, prec)
     ref = ctx.tonal(c)[2]
-    ctx.set_tuning(acum.MIXED, window=window, max_spread=spread, max_tile=32)
+    keep = ctx.get_tuning(prec)
+    ctx.set_tuning(prec, window=window, max_spread=spread, max_tile=tile)
     try:
         got = ctx.tonal(c)[2]
     finally:
-        ctx.set_tuning(acum.MIXED, window=64, max_spread=8, max_tile=256)
-    assert O.rel_l2(got, ref) < 1e-5
+        ctx.set_tuning(prec, window=keep.window, max_spread=keep.max_spread, max_tile=keep.max_tile)
+    assert O.rel_l2(got, ref) < (1e-5 if prec == acum.MIXED else 1e-12)
+
+
+def test_batch_mixes_still_and_moving_cases(ctx):
+    """A hover case (still air) and a forward-flight case run through the two kernel variants in
+    one batch and give the same results as alone."""
+    cases = [small_case(1, acum.MIXED)[1], small_case(2, acum.MIXED)[1], small_case(1, acum.MIXED, nTime=20)[1]]
+    assert cases[1].vInf[0] != 0.0 and cases[0].vInf[0] == 0.0
+    batch = ctx.tonal_batch(cases)
+    for c, b in zip(cases, batch):
+        s = ctx.tonal(c)
+        for x, y in zip(s, b):
+            assert np.array_equal(x, y)
 
 
 def test_errors_are_returned(ctx):
