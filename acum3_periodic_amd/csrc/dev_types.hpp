@@ -10,17 +10,17 @@ namespace acum {
 // time and jt = j (dTau oM / dPsi), the step's emission time in bins.  AttrT holds the
 // observer-independent factors of the Farassat 1A integrand (core.cu:68-154), formed in FP64
 // from the parsed floats and stored in the path's precision R:
-//   v  = V / a0,  vd = dV/dt / a0                         (Mr, dMr of core.cu:93-95)
-//   l  = P n,     ld = dP/dt n + P dn/dt                   (L, dL/dt of core.cu:130-131)
-//   MM = |V|^2 / a0^2,  LM = P (V.n) / a0                  (M^2, L_M of core.cu:94,134)
-//   cT = rhoRef (V.n) dS a0 D1,  cL = dS D1,  D1 = 1/(4 pi a0)   (core.cu:53,123-124,138-141)
+//   v   = V / a0,  vd = dV/dt / a0                        (Mr, dMr of core.cu:93-95)
+//   l   = P n,     ld = dP/dt n + P dn/dt                  (L, dL/dt of core.cu:130-131)
+//   aMM = a0 M^2 = |V|^2 / a0,  aLM = a0 L_M = P (V.n)     (core.cu:94,134)
+//   cT  = rhoRef (V.n) dS a0 D1,  cL = dS D1,  D1 = 1/(4 pi a0)   (core.cu:53,123-124,138-141)
 struct PosRec {
   double x, y, z, jt;
 };  // 32 bytes
 template <typename R>
 struct AttrT {
   R v[3], vd[3], l[3], ld[3];
-  R MM, LM, cT, cL;
+  R aMM, aLM, cT, cL;
 };  // 64 bytes (float), 128 bytes (double)
 
 // Canonical FP64 observer time, in units of the observer-time bin dPsi; the CPU oracle forms

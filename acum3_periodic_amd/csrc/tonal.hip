@@ -23,20 +23,38 @@
 // oracle uses, compiled without contraction, so bin indices agree bit for bit.
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 #include "dev_types.hpp"
 #include "fp64_math.hpp"
 
 namespace acum {
 
-// Records are read through global-address-space pointers with wave-uniform addresses.
+// Records are read with wave-uniform addresses.  Through constant-address-space pointers the
+// reads are scalar loads into SGPRs (the mixed path); through global-address-space pointers the
+// compiler may also use vector loads, which keeps the larger FP64 records out of the SGPR file.
 template <typename T>
 using cptr = const __attribute__((address_space(1))) T*;
+template <typename T>
+using kptr = const __attribute__((address_space(4))) T*;
 
 template <typename T>
 __device__ __forceinline__ T load_words(cptr<T> p) {
   static_assert(sizeof(T) % 4 == 0, "record must be whole dwords");
   constexpr int n = (int)(sizeof(T) / 4);
   cptr<uint32_t> w = (cptr<uint32_t>)p;
+  uint32_t t[n];
+#pragma unroll
+  for (int i = 0; i < n; i++) t[i] = w[i];
+  T r;
+  __builtin_memcpy(&r, t, sizeof(T));
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T load_words(kptr<T> p) {
+  static_assert(sizeof(T) % 4 == 0, "record must be whole dwords");
+  constexpr int n = (int)(sizeof(T) / 4);
+  kptr<uint32_t> w = (kptr<uint32_t>)p;
   uint32_t t[n];
 #pragma unroll
   for (int i = 0; i < n; i++) t[i] = w[i];
@@ -72,28 +90,37 @@ __device__ __forceinline__ float fmaT(float a, float b, float c) { return __buil
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcpT(double x) { return rcp_nr(x); }
+__device__ __forceinline__ float maxT(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double maxT(double a, double b) { return __builtin_fmax(a, b); }
 
 template <bool MIXED>
 struct Prec {
   using R = double;
   using R2 = double2;
+  template <typename T>
+  using ptr = cptr<T>;
 };
 template <>
 struct Prec<true> {
   using R = float;
   using R2 = float2;
+  template <typename T>
+  using ptr = kptr<T>;
 };
 
 // Farassat 1A integrand of the impermeable surface (core.cu:68-154): rho = 1, U = V and the
 // dotted normal velocities forced to zero (core.cu:117-118), so
 //   pThick = rhoRef Vn dS a0 (dMr T2 + a0 (Mr - M^2) T4)
 //   pLoad  = dS (dLr T1 + Lr dMr T2 + a0 (Lr - L_M) T3 + a0 Lr (Mr - M^2) T4)
-// with T1 = D1/(r (1-Mr)^2), T2 = T1/(1-Mr), T3 = T1/r, T4 = T3/(1-Mr).  With q = 1/(1-Mr),
-// T = q^2 / r and A = a0 / r this is
-//   pThick = cT T q (dMr + A (Mr - M^2))
-//   pLoad  = cL T (dLr + Lr q dMr + A ((Lr - L_M) + Lr q (Mr - M^2)))
-// (cT, cL carry the observer-independent factors, see AttrT).  d is obs - src; `del` is r in
-// still air and t = r / a0 in a moving medium, where r_vec = d - vinf t (core.cu:86).
+// with T1 = D1/(r (1-Mr)^2), T2 = T1/(1-Mr), T3 = T1/r, T4 = T3/(1-Mr).  With the record
+// products (AttrT) dotted into r_vec -- Mr = dV/r, dMr = dVd/r, Lr = dL/r, dLr = dLd/r for
+// dV = v.r_vec, dVd = vd.r_vec, dL = l.r_vec, dLd = ld.r_vec -- and g = 1/(r (1 - Mr)) =
+// 1/(r - dV), every 1/r cancels:
+//   pThick = cT g^3 (r (dVd - a0 M^2) + a0 dV)
+//   pLoad  = cL g^2 (dL g (dVd - a0 M^2 + a0) + dLd - a0 L_M)
+// (using 1 + g dV = r g), one reciprocal per eval.  d is obs - src; `del` is r in still air and
+// t = r / a0 in a moving medium, where r_vec = d - vinf t (core.cu:86).  `bad` marks r <= 0 or
+// 1 - Mr <= 0 (and NaN).
 template <typename R, bool MOVING>
 __device__ __forceinline__ void integrand(const AttrT<R>& q, R dx, R dy, R dz, R del, R a0,
                                           R vx, R vy, R vz, R* pth, R* plo, bool* bad) {
@@ -113,19 +140,13 @@ __device__ __forceinline__ void integrand(const AttrT<R>& q, R dx, R dy, R dz, R
   const R dVd = fmaT(q.vd[2], rz, fmaT(q.vd[1], ry, q.vd[0] * rx));
   const R dL = fmaT(q.l[2], rz, fmaT(q.l[1], ry, q.l[0] * rx));
   const R dLd = fmaT(q.ld[2], rz, fmaT(q.ld[1], ry, q.ld[0] * rx));
-  const R ir = rcpT(r);
-  const R Mr = dV * ir, Mdr = dVd * ir, Lr = dL * ir, Ldr = dLd * ir;
-  const R om = (R)1 - Mr;
-  const R qq = rcpT(om);
-  const R T = ir * qq * qq;
-  const R mm = Mr - q.MM;
-  const R A = a0 * ir;
-  *pth = (q.cT * T) * (qq * fmaT(A, mm, Mdr));
-  const R Lq = Lr * qq;
-  R s = fmaT(A, fmaT(Lq, mm, Lr - q.LM), Ldr);
-  s = fmaT(Lq, Mdr, s);
-  *plo = (q.cL * T) * s;
-  *bad = !(r > (R)0) || !(om > (R)0);
+  const R rm = r - dV;                // r (1 - Mr)
+  const R g = rcpT(rm);
+  const R g2 = g * g;
+  const R u = dVd - q.aMM;
+  *pth = (q.cT * g2) * (g * fmaT(r, u, a0 * dV));
+  *plo = (q.cL * g2) * fmaT(dL * g, u + a0, dLd - q.aLM);
+  *bad = !(r > (R)0) || !(rm > (R)0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -133,25 +154,31 @@ __device__ __forceinline__ void integrand(const AttrT<R>& q, R dx, R dy, R dz, R
 //
 // Per eval (element e, step k, observer = lane) with tau_prev, tau, tau_next the observer times
 // of steps k-1, k, k+1 (wrapped by -nT / +nT at the ends of the period, core.cu:173-194):
-//   ic = floor(tau); nd = ic - floor(tau_prev) rising bins ic, ic-1, ... with weights
-//   (b - tau_prev)/(tau - tau_prev) = 1 - (fr + i)/(tau - tau_prev), fr = tau - ic;
-//   nu = ceil(tau_next) - 1 - ic falling bins ic+1, ic+2, ... with weights
+//   ic = floor(tau); the rising bins ic, ic-1, ... are the integers in (tau_prev, tau] with
+//   weights (b - tau_prev)/(tau - tau_prev) = 1 - (fr + i)/(tau - tau_prev), fr = tau - ic; the
+//   falling bins ic+1, ic+2, ... are the integers in (tau, tau_next) with weights
 //   (tau_next - b)/(tau_next - tau) = 1 - (i + 1 - fr)/(tau_next - tau).
-// Bin counts and fr come from the FP64 observer times; the weights are formed in R.
+// The footprint ic-1 .. ic+2 takes the two nearest bins of each side with the weights clamped at
+// zero: a weight is positive exactly when its bin lies inside the tent, so the clamp selects the
+// tent's bins (up to the rounding of the weight itself, below 1e-6 in the mixed path).  Sides
+// longer than two bins go to a wave-uniform tail that counts their bins from the FP64 times.
 template <bool MIXED, bool MOVING, int W>
 __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
                                                         const Tile* __restrict__ tiles,
                                                         const Item* __restrict__ items, int nItems,
                                                         int* __restrict__ counter,
                                                         uint32_t* __restrict__ flags_out) {
-  using R = typename Prec<MIXED>::R;
-  using R2 = typename Prec<MIXED>::R2;
+  using P = Prec<MIXED>;
+  using R = typename P::R;
+  using R2 = typename P::R2;
   using Attr = AttrT<R>;
+  using PosPtr = typename P::template ptr<PosRec>;
+  using AttrPtr = typename P::template ptr<Attr>;
   // win[slot][lane] = (thickness, loading) partial of one observer-time bin
   __shared__ R2 win[W * 64];
   const int lane = threadIdx.x;
   for (int s = 0; s < W; s++) win[s * 64 + lane] = R2{(R)0, (R)0};
-  uint32_t flags = 0;
+  uint64_t badmask = 0, winmask = 0, sweepmask = 0;  // lanes that raised each error
   for (;;) {
     int it = 0;
     if (lane == 0) it = atomicAdd(counter, 1);
@@ -166,8 +193,8 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
     const int oc = active ? o : nObs - 1;
     const double ox = Cp->obs[3 * oc], oy = Cp->obs[3 * oc + 1], oz = Cp->obs[3 * oc + 2];
     R* slab = (R*)Cp->slab + ((int64_t)item.group * nObs + oc) * (int64_t)nT * 2;
-    cptr<PosRec> pos = (cptr<PosRec>)(uintptr_t)Cp->pos;
-    cptr<Attr> attr = (cptr<Attr>)(uintptr_t)Cp->attr;
+    const PosPtr pos = (PosPtr)(uintptr_t)Cp->pos;
+    const AttrPtr attr = (AttrPtr)(uintptr_t)Cp->attr;
     const R a0 = (R)k.a0, vx = (R)k.vx, vy = (R)k.vy, vz = (R)k.vz;
     const double dnT = (double)nT;
     for (int ti = item.tile0; ti < item.tile0 + item.ntile; ti++) {
@@ -185,8 +212,8 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
           base = (int)floor(tb) - tl.spread - 1;
         }
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
-          cptr<PosRec> pe = pos + e * nT;
-          cptr<Attr> ae = attr + e * nT;
+          const PosPtr pe = pos + e * nT;
+          const AttrPtr ae = attr + e * nT;
           // state at the start of the pass: tau(k0-1), tau(k0)
           double t_prev, t_cur, dl;
           {
@@ -204,78 +231,76 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
             dzc = (R)dz;
             delc = (R)dl;
           }
-          if (!(fabs(t_prev) < 1.0e9) || !(fabs(t_cur) < 1.0e9)) flags |= kFlagNumeric;
-          int ic_p = (int)floor(t_prev);
           double fl_c = floor(t_cur);
-          int ic = (int)fl_c;
+          R idp = rcpT((R)(t_cur - t_prev));  // 1 / (tau - tau_prev), carried from step to step
           // record pipeline: positions one step ahead of the solve, attributes one step ahead
           // of the integrand, each with one load in flight
           PosRec pn = load_words<PosRec>(pe + kn0);
           Attr ac = load_words<Attr>(ae + k0);
-          for (int kk = k0; kk < kend; kk++) {
+          auto step = [&](int kk, bool wrap) __attribute__((always_inline)) {
             const int kn = (kk + 1 < nT) ? kk + 1 : 0;
             const int knn = (kn + 1 < nT) ? kn + 1 : 0;
             const PosRec pnn = load_words<PosRec>(pe + knn);
             const Attr an = load_words<Attr>(ae + kn);
             // the footprint is known before the solve: start its window reads now
-            int sb = ic - 1 - base;
-            const bool inrange = sb >= 0 && sb + 3 < W;
-            sb = inrange ? sb : 0;
-            R2* wp = &win[sb * 64 + lane];
+            const int ic = (int)fl_c;
+            const int sb = ic - 1 - base;
+            const bool inrange = (unsigned)sb <= (unsigned)(W - 4);
+            R2* wp = &win[(inrange ? sb : 0) * 64 + lane];
             R2 v0 = wp[0], v1 = wp[64], v2 = wp[128], v3 = wp[192];
+            // keep the record loads and window reads at the top of the step: their latency then
+            // overlaps the solve and the integrand instead of the window update
+            __builtin_amdgcn_sched_barrier(0);
             // observer time of the next step
             const double dxn = ox - pn.x, dyn = oy - pn.y, dzn = oz - pn.z;
             double dln;
             double t_next = obs_time<MOVING>(k, pn.jt, dxn, dyn, dzn, &dln);
-            if (kk == nT - 1) t_next = t_next + dnT;
+            if (wrap) t_next = t_next + dnT;
             // integrand of this step
             R pth, plo;
             bool bad;
             integrand<R, MOVING>(ac, dxc, dyc, dzc, delc, a0, vx, vy, vz, &pth, &plo, &bad);
-            // tent
-            const double fl_n = floor(t_next);
-            const int ic_n = (int)fl_n;
-            const int nd = ic - ic_p;
-            const int nu = ic_n - ic - (t_next == fl_n ? 1 : 0);
-            const bool ok = !bad && fabs(t_next) < 1.0e9;
-            if (!ok) flags |= kFlagNumeric;
-            const R pthv = ok ? pth : (R)0, plov = ok ? plo : (R)0;
+            badmask |= __ballot(bad);
+            winmask |= __ballot(!inrange);
+            // tent weights of the footprint
             const R fr = (R)(t_cur - fl_c);
-            const R idp = rcpT((R)(t_cur - t_prev));
-            const R idn = rcpT((R)(t_next - t_cur));
-            const R wr0 = fmaT(-fr, idp, (R)1);       // bin ic
+            const R dp = (R)(t_cur - t_prev), dn = (R)(t_next - t_cur);
+            const R idn = rcpT(dn);
+            const R wr0 = fmaT(-fr, idp, (R)1);        // bin ic
             const R wf0 = fmaT(fr - (R)1, idn, (R)1);  // bin ic+1
-            const R w1 = nd >= 1 ? wr0 : (R)0;
-            const R w0 = nd >= 2 ? wr0 - idp : (R)0;   // bin ic-1
-            const R w2 = nu >= 1 ? wf0 : (R)0;
-            const R w3 = nu >= 2 ? wf0 - idn : (R)0;   // bin ic+2
-            if (!inrange && (nd | nu) != 0) flags |= kFlagWindow;
-            v0.x = fmaT(pthv, w0, v0.x);
-            v0.y = fmaT(plov, w0, v0.y);
-            v1.x = fmaT(pthv, w1, v1.x);
-            v1.y = fmaT(plov, w1, v1.y);
-            v2.x = fmaT(pthv, w2, v2.x);
-            v2.y = fmaT(plov, w2, v2.y);
-            v3.x = fmaT(pthv, w3, v3.x);
-            v3.y = fmaT(plov, w3, v3.y);
+            const R w0 = maxT(wr0 - idp, (R)0);        // bin ic-1
+            const R w1 = maxT(wr0, (R)0);
+            const R w2 = maxT(wf0, (R)0);
+            const R w3 = maxT(wf0 - idn, (R)0);        // bin ic+2
+            v0.x = fmaT(pth, w0, v0.x);
+            v0.y = fmaT(plo, w0, v0.y);
+            v1.x = fmaT(pth, w1, v1.x);
+            v1.y = fmaT(plo, w1, v1.y);
+            v2.x = fmaT(pth, w2, v2.x);
+            v2.y = fmaT(plo, w2, v2.y);
+            v3.x = fmaT(pth, w3, v3.x);
+            v3.y = fmaT(plo, w3, v3.y);
             wp[0] = v0;
             wp[64] = v1;
             wp[128] = v2;
             wp[192] = v3;
-            const bool tail = nd > 2 || nu > 2;
-            if (wave_any(tail)) {  // tents wider than two bins a side
+            // a side longer than two bins can hold a third bin
+            const bool tail = dp > (R)1.999 || dn > (R)1.999;
+            if (wave_any(tail)) {
               if (tail) {
-                if (nd > W || nu > W) flags |= kFlagSweep;
+                const int nd = ic - (int)floor(t_prev);
+                const int nu = (int)ceil(t_next) - 1 - ic;
+                if (nd > W || nu > W) sweepmask |= 1;
                 const int nd_ = min(nd, W), nu_ = min(nu, W);
                 auto put = [&](int b, R w) {
                   const int slot = b - base;
                   if (slot < 0 || slot >= W) {
-                    flags |= kFlagWindow;
+                    winmask |= 1;
                     return;
                   }
                   R2 v = win[slot * 64 + lane];
-                  v.x = fmaT(pthv, w, v.x);
-                  v.y = fmaT(plov, w, v.y);
+                  v.x = fmaT(pth, w, v.x);
+                  v.y = fmaT(plo, w, v.y);
                   win[slot * 64 + lane] = v;
                 };
                 for (int j = 2; j < nd_; j++) put(ic - j, fmaT(-(fr + (R)j), idp, (R)1));
@@ -284,16 +309,19 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
             }
             t_prev = t_cur;
             t_cur = t_next;
-            ic_p = ic;
-            ic = ic_n;
-            fl_c = fl_n;
+            idp = idn;
+            fl_c = floor(t_next);
             dxc = (R)dxn;
             dyc = (R)dyn;
             dzc = (R)dzn;
             delc = (R)dln;
             pn = pnn;
             ac = an;
-          }
+          };
+          // the last step of the period wraps tau_next by +nT; it is peeled off the loop
+          const int kstop = (kend == nT) ? nT - 1 : kend;
+          for (int kk = k0; kk < kstop; kk++) step(kk, false);
+          if (kend == nT) step(nT - 1, true);
         }
         // add the window into this item's slab row and clear it (untouched slots hold zeros)
         int b = base % nT;
@@ -312,7 +340,9 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
       }
     }
   }
-  if (flags) atomicOr(flags_out, flags);
+  const uint32_t flags = (badmask ? kFlagNumeric : 0u) | (winmask ? kFlagWindow : 0u) |
+                         (sweepmask ? kFlagSweep : 0u);
+  if (flags && lane == 0) atomicOr(flags_out, flags);
 }
 
 // Ordered reduction of the partial slabs: out(b, o) = sum_g slab[g][o][b], FP64 Kahan.
@@ -398,8 +428,8 @@ __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __re
       Vn += Vi * ni;
       VV += Vi * Vi;
     }
-    o.MM = (R)(VV * inv_a0 * inv_a0);
-    o.LM = (R)(P * Vn * inv_a0);
+    o.aMM = (R)(VV * inv_a0);
+    o.aLM = (R)(P * Vn);
     o.cT = (R)(rhoRef * Vn * ds * a0 * D1);
     o.cL = (R)(ds * D1);
     attr[q] = o;

@@ -9,15 +9,23 @@ element data is resident in HBM when the timed region starts.
 
 A step is one full pass of the mixed FP32/FP64 path: clear partials, eval + binning kernel,
 ordered Kahan reduction to pT/pL/pA.  The plain FP64 path is timed the same way and reported
-beside it.  Accuracy of the timed outputs is checked against the FP64 CPU oracle on the first
-observers (that oracle run is also the CPU baseline).
+beside it, and the broadband (BPM) path is timed on its own workload (4 x 60 sections, 360
+steps, 1024 observers, 34 bands).  Accuracy of the timed outputs is checked against the FP64 CPU
+oracle on the first observers (that oracle run is also the CPU baseline).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--oracle-obs N] [--no-fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--oracle-obs N] [--no-fp64] [--no-bpm]
+    python bench.py --summarize-pmc FETCH_DIR WRITE_DIR   # HBM traffic summary into profiles/
+
+Roofline of the eval kernel: 100 FLOP per eval (70 FP32 + 30 FP64 in the mixed path) against
+the MI355X vector peaks; `traffic` is the kernel's measured HBM bytes per launch, read from the
+committed rocprofv3 counter summary (profiles/tonal_eval_c2_hbm_traffic.md) when present.
 """
 import argparse
 import ctypes as C
+import glob
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,6 +38,9 @@ FLOP_PER_EVAL = 100.0          # algorithmic count per (element, step, observer)
 FLOP_FP32, FLOP_FP64 = 70.0, 30.0
 PEAK_FP32, PEAK_FP64 = 157.3e12, 78.6e12   # MI355X vector peaks, FLOP/s
 HBM_PEAK = 8.0e12                           # B/s
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "tonal_eval_c2_hbm_traffic.md")
+EVAL_KERNEL = "tonal_eval_kernel"
+CALIB_BYTES = 1 << 30
 
 
 def mixed_peak():
@@ -37,26 +48,177 @@ def mixed_peak():
     return FLOP_PER_EVAL / (FLOP_FP32 / PEAK_FP32 + FLOP_FP64 / PEAK_FP64)
 
 
+def copy_calibration():
+    """A plain device-to-device copy of CALIB_BYTES (reads and writes CALIB_BYTES) through the HIP
+    runtime, for scaling the FETCH_SIZE / WRITE_SIZE counters; runs only when profiling."""
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    ok = hip.hipMalloc(C.byref(src), C.c_size_t(CALIB_BYTES)) == 0
+    ok = ok and hip.hipMalloc(C.byref(dst), C.c_size_t(CALIB_BYTES)) == 0
+    ok = ok and hip.hipMemset(src, 1, C.c_size_t(CALIB_BYTES)) == 0
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    ok = ok and hip.hipMemcpy(dst, src, C.c_size_t(CALIB_BYTES), 3) == 0  # device to device
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if not ok:
+        raise RuntimeError("calibration copy failed")
+    print(f"calibration copy of {CALIB_BYTES} bytes done", file=sys.stderr, flush=True)
+
+
+def read_traffic():
+    """Bytes per eval-kernel launch from the committed counter summary, or None."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(TRAFFIC_SUMMARY).read())
+    return float(m.group(1)) if m else None
+
+
+def summarize_pmc(fetch_dir, write_dir):
+    """Turns two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) of
+    `bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1 --copy-calibration`
+    into the traffic summary, scaling each counter by its reading for the calibration copy."""
+    import csv
+
+    def load(d, counter):
+        per = {}  # dispatch id -> [kernel name, summed value]
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(f)):
+                if r.get("Counter_Name") == counter:
+                    e = per.setdefault(int(r["Dispatch_Id"]), [r["Kernel_Name"], 0.0])
+                    e[1] += float(r["Counter_Value"])
+        if not per:
+            raise SystemExit(f"no {counter} rows under {d}")
+        return [tuple(per[k]) for k in sorted(per)]
+
+    out = {}
+    for name, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        rows = load(d, name)
+        # the calibration copy is the first copy dispatch of the process (before any staging)
+        copy = [v for k, v in rows if "copy" in k.lower()]
+        ev = [v for k, v in rows if EVAL_KERNEL in k]
+        if not copy or not ev:
+            raise SystemExit(f"{name}: calibration copy or eval kernel missing in {d}")
+        copy_kib = copy[0]
+        scale = (CALIB_BYTES / 1024.0) / copy_kib
+        out[name] = dict(copy_kib=copy_kib, scale=scale, eval_kib=float(np.mean(ev)),
+                         eval_bytes=float(np.mean(ev)) * 1024.0 * scale)
+    total = out["FETCH_SIZE"]["eval_bytes"] + out["WRITE_SIZE"]["eval_bytes"]
+    lines = [
+        "# HBM traffic of the tonal eval kernel: four-blade rotor (4 x 20,000 elements, 1024 steps,",
+        "# 1024 observers), mixed path, one launch",
+        "",
+        "Commands (repository root `R` on the GPU host, one counter per run):",
+        "",
+        "```",
+        "cd /tmp && export TMPDIR=/tmp",
+        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d $R/gpurun_out/pmcF -o p -- \\",
+        "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1 --copy-calibration",
+        "rocprofv3 --pmc WRITE_SIZE --output-format csv -d $R/gpurun_out/pmcW -o p -- \\",
+        "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1 --copy-calibration",
+        "python3 $R/bench.py --summarize-pmc $R/gpurun_out/pmcF $R/gpurun_out/pmcW",
+        "```",
+        "",
+        f"Calibration: a device copy of {CALIB_BYTES} bytes (reads and writes that many bytes).",
+        "",
+        "| counter | copy (KiB) | scale (bytes/KiB-count) | eval kernel (KiB) | eval kernel (bytes) |",
+        "|---|---|---|---|---|",
+    ]
+    for name in ("FETCH_SIZE", "WRITE_SIZE"):
+        o = out[name]
+        lines.append(f"| {name} | {o['copy_kib']:.0f} | {o['scale'] * 1024:.4f} | {o['eval_kib']:.0f} | "
+                     f"{o['eval_bytes']:.4g} |")
+    lines += ["", f"traffic_bytes_per_launch: {total:.6g}", ""]
+    with open(TRAFFIC_SUMMARY, "w") as f:
+        f.write("\n".join(lines))
+    print("\n".join(lines))
+    return 0
+
+
+def bench_bpm(acum, O, ctx, steps):
+    """The broadband path on the c3 workload: GPU kernels and the whole call, and the
+    reference's own single-threaded findBBnoise (oracle/_ref) on a bounded observer sample."""
+    run = acum.Run.synth(acum.synth_params(3))
+    c = run.bpm_case()
+    ctx.bpm(c)  # warm
+    gpu_ms, call_ms = [], []
+    for _ in range(max(1, steps)):
+        t = time.perf_counter()
+        spl, oa = ctx.bpm(c)
+        call_ms.append((time.perf_counter() - t) * 1e3)
+        tm = ctx.timing()
+        gpu_ms.append(tm.prep_ms + tm.eval_ms + tm.reduce_ms)
+    nsrc = int(np.ctypeslib.as_array(c.nsect, shape=(c.nSurf_BB,)).sum()) * c.nTime
+    evals = nsrc * c.nObs
+    band_evals = evals * c.nf2
+    g = float(np.median(gpu_ms))
+    out = dict(workload=f"BPM broadband: {c.nSurf_BB} x {nsrc // c.nSurf_BB // c.nTime} sections, "
+                        f"{c.nTime} steps, {c.nObs} observers, {c.nf2} bands",
+               src_obs_time_evals=evals, band_evals=band_evals, gpu_ms=round(g, 3),
+               call_ms=round(float(np.median(call_ms)), 2),
+               value=round(evals / (g * 1e-3) / 1e9, 3), unit="G src-obs-time evals/s",
+               band_evals_per_s=round(band_evals / (g * 1e-3) / 1e9, 2))
+    # accuracy on the first observers against the FP64 oracle
+    nob = 8
+    ospl, ooa = O.bpm_f64(c, 0, nob)
+    out["max_dspl_db_vs_oracle"] = float(np.abs(spl[:nob].astype(np.float64) - ospl).max())
+    out["max_doaspl_db_vs_oracle"] = float(np.abs(oa[:nob] - ooa).max())
+    if O.ref_available():
+        nref = 2
+        devnull = os.open(os.devnull, os.O_WRONLY)
+        saved = os.dup(1)
+        sys.stdout.flush()
+        os.dup2(devnull, 1)  # the reference prints progress lines
+        try:
+            t = time.perf_counter()
+            O.ref_bpm(c, 0, nref)
+            dt = time.perf_counter() - t
+        finally:
+            os.dup2(saved, 1)
+            os.close(devnull)
+            os.close(saved)
+        out["cpu_baseline"] = dict(value=round(nsrc * nref / dt / 1e9, 6), unit="G src-obs-time evals/s",
+                                   cores=1, kind="reference",
+                                   sample=f"findBBnoise built from the reference sources, first {nref} "
+                                          f"observers ({nsrc * nref:.3g} evals, {dt:.1f} s)")
+        out["vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--oracle-obs", type=int, default=32)
+    ap.add_argument("--oracle-obs", type=int, default=64)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-bpm", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay the pass as a captured HIP graph")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain device copy first (for scaling HBM counters)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     args = ap.parse_args()
+    if args.summarize_pmc:
+        return summarize_pmc(*args.summarize_pmc)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(observer sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         return 2
 
     import acum3_periodic_amd as acum
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O  # tests/oracle.py: the oracle binding (CPU baseline and accuracy only)
+
+    if args.copy_calibration:
+        copy_calibration()
+
+    def note(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     t0 = time.time()
+    note("synthesising and preprocessing the four-blade rotor")
     run = acum.Run.synth(acum.synth_params(2)).preprocess()
     case = run.tonal_case(acum.MIXED)
-    evals = 0
     E = [case.E[s] for s in range(case.nSurf)]
     evals = sum(E) * case.nTime * case.nObs
     host_prep_s = time.time() - t0
@@ -79,28 +241,30 @@ def main():
         tm = ctx.timing()
         return dt / steps, eval_ms / steps, red_ms / steps, tm
 
+    note("timing the mixed path")
     step_s, eval_ms, red_ms, tm = timed(acum.MIXED, args.steps, args.warmup)
     mixed = ctx.fetch(h, 0, case.nTime, case.nObs)
     partials = tm.n_partials
     items = tm.n_items
+    tune = ctx.get_tuning(acum.MIXED)
 
     fp64 = None
     if not args.no_fp64:
+        note("timing the FP64 path")
         s64, e64, r64, t64 = timed(acum.FP64, max(1, min(args.steps, 3)), 1)
         plain = ctx.fetch(h, 0, case.nTime, case.nObs)
-        fp64 = dict(value=evals / s64 / 1e9, ms_per_step=s64 * 1e3, eval_ms=e64, reduce_ms=r64,
-                    partials=t64.n_partials)
+        fp64 = dict(value=round(evals / s64 / 1e9, 3), ms_per_step=round(s64 * 1e3, 3),
+                    eval_ms=round(e64, 3), reduce_ms=round(r64, 4), partials=t64.n_partials,
+                    roofline_frac=round(FLOP_PER_EVAL * evals / (e64 * 1e-3) / PEAK_FP64, 4))
 
     # CPU baseline + accuracy: FP64 oracle on the first observers (observers are independent)
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle as O  # tests/oracle.py: the oracle binding (CPU baseline leg only)
     nob = max(1, min(args.oracle_obs, case.nObs))
     sub = acum.TonalCase()
     C.pointer(sub)[0] = case
     sub.nObs = nob
     sub.precision = acum.FP64
-    threads = len(os.sched_getaffinity(0))
-    threads = min(threads, 16)
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    note(f"CPU oracle on {nob} observers")
     t = time.perf_counter()
     oT, oL, oA = O.tonal_f64(sub, threads)
     cpu_s = time.perf_counter() - t
@@ -114,6 +278,11 @@ def main():
         acc["fp64_vs_oracle_rel_l2_pA"] = O.rel_l2(plain[2][:, :nob], oA)
         d = np.abs(acum.oaspl(mixed[2]) - acum.oaspl(plain[2]))
         acc["mixed_vs_fp64_all_obs_max_doaspl_db"] = float(d.max())
+
+    bpm = None
+    if not args.no_bpm:
+        note("broadband path")
+        bpm = bench_bpm(acum, O, ctx, args.steps)
 
     flops = FLOP_PER_EVAL * evals
     achieved = flops / (eval_ms * 1e-3)
@@ -136,11 +305,13 @@ def main():
         "config": {"workload": "tonal Farassat 1A, 4-blade rotor in forward flight: 4 x 20,000 "
                                "elements, 1024 steps, 1024 observers (8.39e10 evals)",
                    "evals_per_step": evals, "precision": "mixed", "graph": bool(args.graph),
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU",
+                   "tile": {"window": tune.window, "max_spread": tune.max_spread,
+                            "max_tile": tune.max_tile}},
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": round(peak / 1e12, 3), "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None,
-                     "kernel": "tonal_eval_kernel<mixed>", "eval_ms": round(eval_ms, 3),
+                     "frac": round(achieved / peak, 4), "traffic": read_traffic(),
+                     "kernel": "tonal_eval_kernel<mixed, moving>", "eval_ms": round(eval_ms, 3),
                      "flop_per_eval": FLOP_PER_EVAL},
         "cpu_baseline": {"value": round(cpu_evals / cpu_s / 1e9, 4), "unit": "G src-obs-time evals/s",
                          "cores": threads, "kind": "port",
@@ -153,6 +324,7 @@ def main():
         "fp64_path": fp64,
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
         "accuracy": acc,
+        "bpm": bpm,
         "work_items": items,
         "host_prep_s": round(host_prep_s, 2),
         "stage_s": round(stage_s, 2),
