@@ -15,6 +15,8 @@ oracle on the first observers (that oracle run is also the CPU baseline).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--oracle-obs N] [--no-fp64] [--no-bpm]
     python bench.py --summarize-pmc FETCH_DIR WRITE_DIR   # HBM traffic summary into profiles/
+    python bench.py --summarize-sq DIR                     # instruction mix into profiles/
+    python bench.py --summarize-stats DIR                  # kernel times into profiles/
 
 Roofline of the eval kernel: 100 FLOP per eval (70 FP32 + 30 FP64 in the mixed path) against
 the MI355X vector peaks; `traffic` is the kernel's measured HBM bytes per launch, read from the
@@ -135,6 +137,69 @@ def summarize_pmc(fetch_dir, write_dir):
     return 0
 
 
+SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_SMEM", "SQ_INSTS_LDS",
+               "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_INST_ANY")
+
+
+def summarize_sq(d):
+    """Per wave-eval instruction counts of the eval kernel from one rocprofv3 --pmc run with the
+    SQ_COUNTERS over `bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1`."""
+    import csv
+    tot = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(f)):
+            if EVAL_KERNEL in r["Kernel_Name"]:
+                tot[r["Counter_Name"]] = tot.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    wave_evals = 80000 * 1024 * 1024 / 64  # c2: elements x steps x observers / 64 lanes
+    lines = ["# Instruction mix of the tonal eval kernel (mixed path): four-blade rotor, 4 x 20,000",
+             "# elements, 1024 steps, 1024 observers, one launch", "",
+             "Command (repository root `R` on the GPU host):", "", "```",
+             "cd /tmp && export TMPDIR=/tmp",
+             "rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d $R/gpurun_out/pmcSQ -o p -- \\",
+             "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1",
+             "python3 $R/bench.py --summarize-sq $R/gpurun_out/pmcSQ", "```", "",
+             f"Per wave-eval = counter / {wave_evals:.4g} (8.39e10 evals / 64 lanes). WAVE_CYCLES and",
+             "WAIT_INST_ANY count in units of 4 cycles.", "",
+             "| counter | total | per wave-eval |", "|---|---|---|"]
+    for k in SQ_COUNTERS:
+        if k in tot:
+            lines.append(f"| {k} | {tot[k]:.4g} | {tot[k] / wave_evals:.2f} |")
+    out = os.path.join(ROOT, "profiles", "tonal_eval_c2_sq_counters.md")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+    return 0
+
+
+def summarize_stats(d):
+    """Kernel time table of one rocprofv3 --kernel-trace --stats run of
+    `bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2` (both tonal precisions)."""
+    import csv
+    files = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no kernel_stats.csv under {d}")
+    rows = list(csv.DictReader(open(files[0])))
+    lines = ["# Kernel statistics: tonal path, four-blade rotor (4 x 20,000 elements, 1024 steps,",
+             "# 1024 observers), mixed and FP64 passes", "",
+             "Command (repository root `R` on the GPU host):", "", "```",
+             "cd /tmp && export TMPDIR=/tmp",
+             "rocprofv3 --kernel-trace --stats --output-format csv -d $R/gpurun_out/prof -o bench -- \\",
+             "    python3 $R/bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2",
+             "python3 $R/bench.py --summarize-stats $R/gpurun_out/prof", "```", "",
+             "`tonal_eval_kernel<true, ...>` is the mixed path, `<false, ...>` the FP64 path.", "",
+             "| kernel | calls | avg ms | min ms | max ms | share |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        n = r["Name"].split("(")[0].replace("void ", "").replace("acum::", "")
+        lines.append(f"| `{n}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.3f} | "
+                     f"{float(r['MinNs']) / 1e6:.3f} | {float(r['MaxNs']) / 1e6:.3f} | "
+                     f"{float(r['Percentage']):.2f}% |")
+    out = os.path.join(ROOT, "profiles", "tonal_c2_kernel_stats.md")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+    return 0
+
+
 def bench_bpm(acum, O, ctx, steps):
     """The broadband path on the c3 workload: GPU kernels and the whole call, and the
     reference's own single-threaded findBBnoise (oracle/_ref) on a bounded observer sample."""
@@ -197,9 +262,15 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy first (for scaling HBM counters)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--summarize-sq", metavar="DIR")
+    ap.add_argument("--summarize-stats", metavar="DIR")
     args = ap.parse_args()
+    if args.summarize_stats:
+        return summarize_stats(args.summarize_stats)
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc)
+    if args.summarize_sq:
+        return summarize_sq(args.summarize_sq)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(observer sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
