@@ -15,7 +15,7 @@
 namespace acum {
 hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
                              const TonalCaseDev* cases, const Tile* tiles, const Item* items,
-                             int nItems, int* counter, uint32_t* flags);
+                             const int* qoff, int* counter, uint32_t* flags);
 hipError_t eval_occupancy(bool mixed, bool moving, int W, int* blocks_per_cu);
 bool eval_window_supported(bool mixed, int W);
 hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* cases, int icase,
@@ -63,6 +63,10 @@ struct Plan {
   void* d_slab = nullptr;
   int n_still = 0;  // items[0, n_still) are still-air cases, the rest moving-medium ones
   int grid[2] = {0, 0};
+  // queue q of launch l (0 still air, 1 moving) holds items [qoff[l][q], qoff[l][q+1]) of that
+  // launch's item range
+  int qoff[2][kQueues + 1] = {};
+  int* d_qoff = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -117,6 +121,7 @@ void free_plan(Plan& p) {
   if (p.graph) (void)hipGraphDestroy(p.graph);
   (void)hipFree(p.d_tiles);
   (void)hipFree(p.d_items);
+  (void)hipFree(p.d_qoff);
   (void)hipFree(p.d_cases);
   (void)hipFree(p.d_slab);
   p = Plan();
@@ -394,16 +399,68 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
       t0 = t1;
     }
   }
-  // still-air items first (one kernel), then moving-medium items (another); longest first in each
-  std::stable_sort(p.items.begin(), p.items.end(), [&](const Item& a, const Item& b) {
-    const int sa = st.cases[a.icase].k.still, sb = st.cases[b.icase].k.still;
-    if (sa != sb) return sa > sb;
-    return a.cost > b.cost;
-  });
-  p.n_still = 0;
-  for (const Item& it : p.items) p.n_still += st.cases[it.icase].k.still ? 1 : 0;
+  // Still-air items first (one launch), then moving-medium items (another).  Within a launch the
+  // tile groups (all observer groups of one group of tiles) are spread over the kQueues queues,
+  // longest first onto the least loaded queue; a queue lists its groups longest first with each
+  // group's items adjacent, so the waves of one XCD work on the same records at the same time.
+  {
+    struct Unit {
+      int icase, group, queue;
+      int64_t cost;
+    };
+    std::map<std::pair<int, int>, int> uid;
+    std::vector<Unit> units;
+    for (const Item& it : p.items) {
+      auto key = std::make_pair(it.icase, it.group);
+      auto f = uid.find(key);
+      if (f == uid.end()) {
+        uid[key] = (int)units.size();
+        units.push_back(Unit{it.icase, it.group, 0, 0});
+        f = uid.find(key);
+      }
+      units[f->second].cost += it.cost;
+    }
+    for (int kind = 0; kind < 2; kind++) {  // 0: still air, 1: moving
+      std::vector<int> ids;
+      for (int u = 0; u < (int)units.size(); u++)
+        if ((st.cases[units[u].icase].k.still ? 0 : 1) == kind) ids.push_back(u);
+      std::stable_sort(ids.begin(), ids.end(),
+                       [&](int a, int b) { return units[a].cost > units[b].cost; });
+      int64_t load[kQueues] = {};
+      for (int u : ids) {
+        int best = 0;
+        for (int q = 1; q < kQueues; q++)
+          if (load[q] < load[best]) best = q;
+        units[u].queue = best;
+        load[best] += units[u].cost;
+      }
+    }
+    std::stable_sort(p.items.begin(), p.items.end(), [&](const Item& a, const Item& b) {
+      const Unit& ua = units[uid[std::make_pair(a.icase, a.group)]];
+      const Unit& ub = units[uid[std::make_pair(b.icase, b.group)]];
+      const int sa = st.cases[a.icase].k.still, sb = st.cases[b.icase].k.still;
+      if (sa != sb) return sa > sb;
+      if (ua.queue != ub.queue) return ua.queue < ub.queue;
+      if (ua.cost != ub.cost) return ua.cost > ub.cost;
+      if (a.icase != b.icase) return a.icase < b.icase;
+      if (a.group != b.group) return a.group < b.group;
+      return a.obs0 < b.obs0;
+    });
+    p.n_still = 0;
+    for (const Item& it : p.items) p.n_still += st.cases[it.icase].k.still ? 1 : 0;
+    for (int l = 0; l < 2; l++) {
+      const int lo = l == 0 ? 0 : p.n_still, hi = l == 0 ? p.n_still : (int)p.items.size();
+      int i = lo;
+      for (int q = 0; q < kQueues; q++) {
+        p.qoff[l][q] = i - lo;
+        while (i < hi && units[uid[std::make_pair(p.items[i].icase, p.items[i].group)]].queue == q) i++;
+      }
+      p.qoff[l][kQueues] = i - lo;
+    }
+  }
   HIPCHK(ctx, hipMalloc(&p.d_tiles, p.tiles.size() * sizeof(Tile)));
   HIPCHK(ctx, hipMalloc(&p.d_items, p.items.size() * sizeof(Item)));
+  HIPCHK(ctx, hipMalloc(&p.d_qoff, sizeof(p.qoff)));
   HIPCHK(ctx, hipMalloc(&p.d_cases, nc * sizeof(TonalCaseDev)));
   HIPCHK(ctx, hipMalloc(&p.d_slab, p.slab_bytes));
   std::vector<TonalCaseDev> hc(nc);
@@ -424,6 +481,7 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
   }
   HIPCHK(ctx, hipMemcpyAsync(p.d_tiles, p.tiles.data(), p.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(p.d_items, p.items.data(), p.items.size() * sizeof(Item), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(p.d_qoff, p.qoff, sizeof(p.qoff), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(p.d_cases, hc.data(), nc * sizeof(TonalCaseDev), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   p.grid[0] = std::min<int>(p.n_still, capacity);
@@ -444,10 +502,11 @@ int enqueue_pass(acum_ctx* ctx, Stage& st, int prec, bool with_events) {
   const int nmov = (int)p.items.size() - p.n_still;
   if (p.n_still > 0)
     HIPCHK(ctx, launch_tonal_eval(mixed, false, p.tune.window, p.grid[0], s, p.d_cases, p.d_tiles,
-                                  p.d_items, p.n_still, ctx->d_counter, ctx->d_flags));
+                                  p.d_items, p.d_qoff, ctx->d_counter, ctx->d_flags));
   if (nmov > 0)
     HIPCHK(ctx, launch_tonal_eval(mixed, true, p.tune.window, p.grid[1], s, p.d_cases, p.d_tiles,
-                                  p.d_items + p.n_still, nmov, ctx->d_counter + 1, ctx->d_flags));
+                                  p.d_items + p.n_still, p.d_qoff + (kQueues + 1),
+                                  ctx->d_counter + kQueues, ctx->d_flags));
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
   for (int c = 0; c < (int)st.cases.size(); c++)
     HIPCHK(ctx, launch_tonal_reduce(mixed, s, p.d_cases, c, (int)st.cases[c].nT, (int)st.cases[c].nObs));
@@ -587,7 +646,7 @@ int acum_ctx_create(acum_ctx** out, int device) {
       delete c;
       return ACUM_ERR_DEVICE;
     }
-  if (hipMalloc(&c->d_counter, 2 * sizeof(int)) != hipSuccess ||
+  if (hipMalloc(&c->d_counter, 2 * kQueues * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_flags, sizeof(uint32_t)) != hipSuccess) {
     delete c;
     return ACUM_ERR_DEVICE;

@@ -73,6 +73,11 @@ struct Item {
   int64_t cost;   // evals, for longest-first ordering
 };
 
+// Work queues of one eval launch: one per XCD (an MI355X has 8), each holding whole tile groups
+// (all observer groups of a group of tiles) so the waves that share those tiles' records share
+// an L2.  A wave starts on its own XCD's queue and then drains the others.
+constexpr int kQueues = 8;
+
 // Error flags raised by the kernels (OR-ed into a device word).
 enum : uint32_t {
   kFlagWindow = 1u,   // a contribution fell outside its tile window (planner bound broken)

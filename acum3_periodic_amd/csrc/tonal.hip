@@ -165,7 +165,8 @@ __device__ __forceinline__ void integrand(const AttrT<R>& q, R dx, R dy, R dz, R
 template <bool MIXED, bool MOVING, int W>
 __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
                                                         const Tile* __restrict__ tiles,
-                                                        const Item* __restrict__ items, int nItems,
+                                                        const Item* __restrict__ items,
+                                                        const int* __restrict__ qoff,
                                                         int* __restrict__ counter,
                                                         uint32_t* __restrict__ flags_out) {
   using P = Prec<MIXED>;
@@ -179,12 +180,20 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
   const int lane = threadIdx.x;
   for (int s = 0; s < W; s++) win[s * 64 + lane] = R2{(R)0, (R)0};
   uint64_t badmask = 0, winmask = 0, sweepmask = 0;  // lanes that raised each error
+  // start on this XCD's queue (hardware register XCC_ID, bits 3:0), then drain the others
+  int q = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & (kQueues - 1));
+  int drained = 0;
   for (;;) {
     int it = 0;
-    if (lane == 0) it = atomicAdd(counter, 1);
+    if (lane == 0) it = atomicAdd(&counter[q], 1);
     it = __builtin_amdgcn_readfirstlane(it);  // lane 0's value, wave-uniform
-    if (it >= nItems) break;
-    const Item item = items[it];
+    const int idx = qoff[q] + it;
+    if (idx >= qoff[q + 1]) {
+      if (++drained == kQueues) break;
+      q = (q + 1) & (kQueues - 1);
+      continue;
+    }
+    const Item item = items[idx];
     const TonalCaseDev* Cp = cases + item.icase;
     const TimeConst k = Cp->k;
     const int nT = k.nT, nObs = Cp->nObs;
@@ -444,8 +453,7 @@ __global__ void fill_zero_kernel(T* __restrict__ p, int64_t n) {
 }
 
 __global__ void reset_counter_kernel(int* c, uint32_t* flags) {
-  c[0] = 0;  // still-air item queue
-  c[1] = 0;  // moving-medium item queue
+  for (int i = 0; i < 2 * kQueues; i++) c[i] = 0;  // still-air queues, then moving-medium queues
   *flags = 0u;
 }
 
@@ -503,16 +511,16 @@ __global__ void tonal_bins_kernel(const PosRec* __restrict__ pos, int64_t nE, Ti
 
 hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
                              const TonalCaseDev* cases, const Tile* tiles, const Item* items,
-                             int nItems, int* counter, uint32_t* flags) {
+                             const int* qoff, int* counter, uint32_t* flags) {
   dim3 b(64);
 #define ACUM_EVAL(M, WW)                                                                       \
   if (mixed == M && W == WW) {                                                                 \
     if (moving)                                                                                \
       hipLaunchKernelGGL((tonal_eval_kernel<M, true, WW>), dim3(grid), b, 0, s, cases, tiles,  \
-                         items, nItems, counter, flags);                                       \
+                         items, qoff, counter, flags);                                       \
     else                                                                                       \
       hipLaunchKernelGGL((tonal_eval_kernel<M, false, WW>), dim3(grid), b, 0, s, cases, tiles, \
-                         items, nItems, counter, flags);                                       \
+                         items, qoff, counter, flags);                                       \
     return hipGetLastError();                                                                  \
   }
   ACUM_EVAL_VARIANTS(ACUM_EVAL)
