@@ -104,6 +104,7 @@ def _load():
         "acum_run_load": (C.c_int, [C.c_char_p, C.POINTER(P)]),
         "acum_run_free": (None, [P]),
         "acum_run_preprocess": (C.c_int, [P]),
+        "acum_preprocess_gpu": (C.c_int, [P, P]),
         "acum_run_inputs": (C.c_int, [P, C.POINTER(Inputs)]),
         "acum_run_tonal_case": (C.c_int, [P, C.c_int32, C.POINTER(TonalCase)]),
         "acum_run_bpm_case": (C.c_int, [P, C.POINTER(BpmCase)]),
@@ -130,7 +131,7 @@ EXPORTS = ("acum_device_count", "acum_ctx_create", "acum_ctx_destroy", "acum_las
            "acum_tonal_batch", "acum_tonal_stage", "acum_tonal_run", "acum_tonal_fetch",
            "acum_tonal_release", "acum_tonal_bins", "acum_get_timing", "acum_get_tuning",
            "acum_set_tuning", "acum_autotune", "acum_bpm", "acum_host_error", "acum_run_load",
-           "acum_run_free", "acum_run_preprocess", "acum_run_inputs", "acum_run_tonal_case",
+           "acum_run_free", "acum_run_preprocess", "acum_preprocess_gpu", "acum_run_inputs", "acum_run_tonal_case",
            "acum_run_bpm_case", "acum_run_write_tonal", "acum_run_write_bb", "acum_run_save",
            "acum_run_surface_raw", "acum_synth_defaults", "acum_synth_batch_case", "acum_synth",
            "acum_oaspl")
@@ -192,6 +193,11 @@ class Run:
 
     def preprocess(self):
         _host_check(lib.acum_run_preprocess(self._h))
+        return self
+
+    def preprocess_gpu(self, ctx):
+        """The same preprocessing on the context's GPU (bit-identical to preprocess())."""
+        ctx._check(lib.acum_preprocess_gpu(ctx._h, self._h))
         return self
 
     def inputs(self):

@@ -97,6 +97,10 @@ void gauss_smooth(double* f, const double* g, int n, int nw, double* out) {
 
 }  // namespace
 
+namespace acum_host {
+void gauss_weights13(double* g) { gauss_weights(g, 13); }
+}  // namespace acum_host
+
 extern "C" int acum_run_preprocess(acum_run* r) {
   if (!r) return fail(ACUM_ERR_ARG, "null run");
   if (r->preprocessed) return fail(ACUM_ERR_ARG, "run already preprocessed");

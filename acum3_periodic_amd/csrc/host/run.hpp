@@ -36,4 +36,6 @@ void derive_inputs(acum_inputs* in, float* dPsi_f);
 const std::vector<float>& reference_bands();
 // Allocates the AoS arrays of a run from nSurf, nTime and E (io.cu:243-261).
 void layout_surfaces(acum_run* r);
+// Normalised 13-point Gaussian of the low-pass filter (mathfunc.cu:350-370), in FP64.
+void gauss_weights13(double* g);
 }  // namespace acum_host

@@ -288,13 +288,35 @@ def main():
 
     t0 = time.time()
     note("synthesising and preprocessing the four-blade rotor")
-    run = acum.Run.synth(acum.synth_params(2)).preprocess()
+    run = acum.Run.synth(acum.synth_params(2))
+    synth_s = time.time() - t0
+    t0 = time.time()
+    run.preprocess()
+    host_prep_s = time.time() - t0
     case = run.tonal_case(acum.MIXED)
     E = [case.E[s] for s in range(case.nSurf)]
     evals = sum(E) * case.nTime * case.nObs
-    host_prep_s = time.time() - t0
 
     ctx = acum.Context(0)
+    prep = None
+    if not args.no_bpm:  # the full measurement also times the GPU preprocessing (next row)
+        note("GPU preprocessing")
+        run2 = acum.Run.synth(acum.synth_params(2))
+        t0 = time.time()
+        run2.preprocess_gpu(ctx)
+        gpu_call_s = time.time() - t0
+        tp = ctx.timing()
+        a = acum.case_arrays(case)
+        b = acum.case_arrays(run2.tonal_case(acum.MIXED))
+        same = all(np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)) for k in ("X", "N", "V", "PR"))
+        nrec = sum(E) * case.nTime
+        prep = dict(records=nrec, host_fp64_s=round(host_prep_s, 3),
+                    host_threads=min(len(os.sched_getaffinity(0)), 16),
+                    gpu_kernel_ms=round(tp.eval_ms, 3), gpu_upload_ms=round(tp.prep_ms, 2),
+                    gpu_download_ms=round(tp.reduce_ms, 2), gpu_call_s=round(gpu_call_s, 3),
+                    kernel_GBps=round(nrec * 92 / (tp.eval_ms * 1e-3) / 1e9, 1),
+                    bit_identical_to_host=bool(same))
+        del run2
     t0 = time.time()
     h = ctx.stage([case])
     stage_s = time.time() - t0
@@ -396,7 +418,9 @@ def main():
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
         "accuracy": acc,
         "bpm": bpm,
+        "preprocess": prep,
         "work_items": items,
+        "synth_s": round(synth_s, 2),
         "host_prep_s": round(host_prep_s, 2),
         "stage_s": round(stage_s, 2),
     }

@@ -168,6 +168,11 @@ int acum_run_load(const char* dir, acum_run** run);
 void acum_run_free(acum_run* run);
 /* preprocess_imper (preprocess.cu:228-413): scaling, periodic fix, derivatives in FP64.      */
 int acum_run_preprocess(acum_run* run);
+/* The same preprocessing on the context's GPU (preprocess.cu:228-413, called at main.cu:88):
+ * the run's element arrays are copied to the device, processed in FP64 with the operations of
+ * acum_run_preprocess (bit-identical results) and copied back.  nTime <= 4096.  Timing:
+ * prep_ms = upload, eval_ms = kernel, reduce_ms = download.                                 */
+int acum_preprocess_gpu(acum_ctx* ctx, acum_run* run);
 int acum_run_inputs(const acum_run* run, acum_inputs* in);
 int acum_run_tonal_case(const acum_run* run, int32_t precision, acum_tonal_case* c);
 int acum_run_bpm_case(const acum_run* run, acum_bpm_case* c);
