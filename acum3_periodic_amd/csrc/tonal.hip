@@ -355,45 +355,84 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
 }
 
 // Ordered reduction of the partial slabs: out(b, o) = sum_g slab[g][o][b], FP64 Kahan.
-// Block 32 x 8 covers 32 bins x 32 observers; LDS transposes to the time-major outputs.
+// A block of 256 threads covers 64 bins x 16 observers: thread (tx, ty) sums bins b0 + 2 tx and
+// b0 + 2 tx + 1 (thickness and loading: two 2-vectors per partial row) of observers o0 + ty and
+// o0 + ty + 8, with eight partial rows in flight; LDS transposes the sums to the time-major
+// outputs, written as runs of 16 observers.
+template <typename A>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  using T = float2;
+};
+template <>
+struct Vec2<double> {
+  using T = double2;
+};
+
 template <typename A>
 __global__ __launch_bounds__(256) void tonal_reduce_kernel(const TonalCaseDev* __restrict__ cases,
                                                            int icase) {
+  using V2 = typename Vec2<A>::T;
+  constexpr int kRows = 8;
   const TonalCaseDev& C = cases[icase];
   const int nT = C.k.nT, nObs = C.nObs, G = C.G;
   const A* slab = (const A*)C.slab;
-  const int b0 = blockIdx.x * 32, o0 = blockIdx.y * 32;
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  __shared__ float sT[32][33], sL[32][33], sA[32][33];
-  for (int i = 0; i < 4; i++) {
-    const int ol = ty + 8 * i, o = o0 + ol, b = b0 + tx;
+  const int b0 = blockIdx.x * 64, o0 = blockIdx.y * 16;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  __shared__ float sT[16][65], sL[16][65], sA[16][65];
+  const int64_t gstride = (int64_t)nObs * nT;  // in V2 units
+  for (int i = 0; i < 2; i++) {
+    const int ol = ty + 8 * i, o = o0 + ol, b = b0 + 2 * tx;
     if (o < nObs && b < nT) {
-      double sT_ = 0.0, cT = 0.0, sL_ = 0.0, cL = 0.0;
-      const A* p = slab + ((int64_t)o * nT + b) * 2;
-      const int64_t gstride = (int64_t)nObs * nT * 2;
-      for (int g = 0; g < G; g++) {
-        const double xT = (double)p[0], xL = (double)p[1];
-        double y = xT - cT, t = sT_ + y;
-        cT = (t - sT_) - y;
-        sT_ = t;
-        y = xL - cL;
-        t = sL_ + y;
-        cL = (t - sL_) - y;
-        sL_ = t;
-        p += gstride;
+      const bool two = b + 1 < nT;
+      const V2* p = (const V2*)slab + (int64_t)o * nT + b;
+      double s[4] = {0.0, 0.0, 0.0, 0.0}, c[4] = {0.0, 0.0, 0.0, 0.0};
+      auto add = [&](int k, double x) {
+        const double y = x - c[k], t = s[k] + y;
+        c[k] = (t - s[k]) - y;
+        s[k] = t;
+      };
+      int g = 0;
+      for (; g + kRows <= G; g += kRows) {
+        V2 v0[kRows], v1[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; r++) {
+          v0[r] = p[(g + r) * gstride];
+          v1[r] = two ? p[(g + r) * gstride + 1] : V2{(A)0, (A)0};
+        }
+#pragma unroll
+        for (int r = 0; r < kRows; r++) {
+          add(0, (double)v0[r].x);
+          add(1, (double)v0[r].y);
+          add(2, (double)v1[r].x);
+          add(3, (double)v1[r].y);
+        }
       }
-      sT[ol][tx] = (float)sT_;
-      sL[ol][tx] = (float)sL_;
-      sA[ol][tx] = (float)(sT_ + sL_);
+      for (; g < G; g++) {
+        const V2 a0 = p[g * gstride];
+        const V2 a1 = two ? p[g * gstride + 1] : V2{(A)0, (A)0};
+        add(0, (double)a0.x);
+        add(1, (double)a0.y);
+        add(2, (double)a1.x);
+        add(3, (double)a1.y);
+      }
+      sT[ol][2 * tx] = (float)s[0];
+      sL[ol][2 * tx] = (float)s[1];
+      sA[ol][2 * tx] = (float)(s[0] + s[1]);
+      sT[ol][2 * tx + 1] = (float)s[2];
+      sL[ol][2 * tx + 1] = (float)s[3];
+      sA[ol][2 * tx + 1] = (float)(s[2] + s[3]);
     }
   }
   __syncthreads();
   for (int i = 0; i < 4; i++) {
-    const int bl = ty + 8 * i, b = b0 + bl, o = o0 + tx;
+    const int idx = threadIdx.x + 256 * i, bl = idx >> 4, ol = idx & 15;
+    const int b = b0 + bl, o = o0 + ol;
     if (b < nT && o < nObs) {
-      C.pT[(int64_t)b * nObs + o] = sT[tx][bl];
-      C.pL[(int64_t)b * nObs + o] = sL[tx][bl];
-      C.pA[(int64_t)b * nObs + o] = sA[tx][bl];
+      C.pT[(int64_t)b * nObs + o] = sT[ol][bl];
+      C.pL[(int64_t)b * nObs + o] = sL[ol][bl];
+      C.pA[(int64_t)b * nObs + o] = sA[ol][bl];
     }
   }
 }
@@ -550,7 +589,7 @@ bool eval_window_supported(bool mixed, int W) {
 
 hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* cases, int icase,
                                int nT, int nObs) {
-  dim3 g((nT + 31) / 32, (nObs + 31) / 32), b(32, 8);
+  dim3 g((nT + 63) / 64, (nObs + 15) / 16), b(256);
   if (mixed)
     hipLaunchKernelGGL(tonal_reduce_kernel<float>, g, b, 0, s, cases, icase);
   else
