@@ -812,10 +812,11 @@ int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* b
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const bool mixed = precision == ACUM_MIXED;
   std::vector<acum_tune> cand;
-  for (int W : {16, 24, 32, 48, 64}) {
+  // the window sets the occupancy and the spread the tile size; the tile length matters little
+  for (int W : {16, 24, 32, 48}) {
     if (!eval_window_supported(mixed, W)) continue;
-    for (int sp : {1, 2, 3, 4, 6})
-      for (int mt : {64, 128, 256}) {
+    for (int sp : {2, 3, 4})
+      for (int mt : {256}) {
         acum_tune t = default_tune(precision);
         t.window = W;
         t.max_spread = std::max(1, sp);

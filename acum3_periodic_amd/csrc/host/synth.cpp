@@ -249,9 +249,14 @@ int acum_synth_batch_case(uint64_t seed, int32_t icase, acum_synth_params* p) {
   p->nTime = g.uniform_int(256, 2048);
   p->obs_layout = 2;
   p->nObs = g.uniform_int(16, 2048);
-  p->obsRadius = p->R * (5.0 + 45.0 * g.uniform());
+  const double rad = 5.0 + 45.0 * g.uniform();
   p->oM = 150.0 + 250.0 * g.uniform();
   p->Minf = 0.2 * g.uniform();
+  // Radius: 1 m, reduced where needed so the advancing blade tip stays below Mach 0.85 (the
+  // Farassat 1A integrand is singular at Mr = 1; at oM = 400 rad/s a 1 m blade is supersonic).
+  p->R = std::min(1.0, (0.85 - p->Minf) * p->a0 / p->oM);
+  p->chord = 0.08 * p->R;
+  p->obsRadius = p->R * rad;
   p->xz_angle = -5.0;
   p->cp_model = 1;
   p->cpNoise = 0.01;

@@ -250,6 +250,123 @@ def bench_bpm(acum, O, ctx, steps):
     return out
 
 
+def note(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_batched(acum, O, args):
+    """The batched workload: 64 seeded rotor cases of uneven size (2-6 blades, 2,000-40,000
+    elements per blade, 256-2048 steps, 16-2048 observers on spheres of 5R-50R, oM 150-400,
+    Minf 0-0.2), run as batches of `--wave` cases per persistent launch (all 64 at once exceed
+    the host and device memory), each batch replayed as a captured HIP graph.  The tile shape
+    comes from the autotuner on the first case; one case per batch is also run alone and must
+    give bit-identical histories."""
+    ctx = acum.Context(0)
+    seed = 31004
+    total_evals = 0
+    gpu_ms = 0.0
+    host_s = 0.0
+    tuned = None
+    checked = 0
+    for w0 in range(0, args.cases, args.wave):
+        idx = list(range(w0, min(args.cases, w0 + args.wave)))
+        t0 = time.time()
+        runs = [acum.Run.synth(acum.batch_params(seed, i)).preprocess() for i in idx]
+        cases = [r.tonal_case(acum.MIXED) for r in runs]
+        host_s += time.time() - t0
+        if tuned is None:
+            h1 = ctx.stage([cases[0]])
+            tuned, tms = ctx.autotune(h1, acum.MIXED)
+            ctx.release(h1)
+            note(f"autotuned tile shape: window {tuned.window}, spread {tuned.max_spread}, "
+                 f"tile {tuned.max_tile} ({tms:.1f} ms on case 0)")
+        h = ctx.stage(cases)
+        ctx.run(h, acum.MIXED, repeats=1, graph=True)  # plan, capture, warm
+        tm = ctx.run(h, acum.MIXED, repeats=2, graph=True)
+        gpu_ms += tm.total_ms
+        ev = sum(sum(c.E[s] for s in range(c.nSurf)) * c.nTime * c.nObs for c in cases)
+        total_evals += ev
+        batch_out = ctx.fetch(h, 0, cases[0].nTime, cases[0].nObs)
+        ctx.release(h)
+        alone = ctx.tonal(cases[0])
+        same = all(np.array_equal(a, b) for a, b in zip(batch_out, alone))
+        if not same:
+            raise SystemExit(f"case {idx[0]}: batched result differs from the case run alone")
+        checked += 1
+        note(f"cases {idx[0]}-{idx[-1]}: {ev:.3g} evals, {tm.total_ms:.1f} ms per pass")
+        del runs, cases
+    value = total_evals / (gpu_ms * 1e-3) / 1e9
+    out = {"metric": "G src-obs-time evals/s (batched, mixed)", "value": round(value, 3),
+           "unit": "G src-obs-time evals/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": f"{args.cases} seeded rotor cases of uneven size in batches of "
+                                  f"{args.wave}, HIP graph replay", "evals": total_evals,
+                      "tile": {"window": tuned.window, "max_spread": tuned.max_spread,
+                               "max_tile": tuned.max_tile}},
+           "gpu_s": round(gpu_ms / 1e3, 3), "host_synth_prep_s": round(host_s, 1),
+           "batches_checked_against_alone": checked}
+    ctx.close()
+    print(json.dumps(out))
+    return 0
+
+
+def bench_stress(acum, O, args):
+    """The stress workload: coaxial counter-rotating 2 x 4 blades, 50,000 elements per blade,
+    2048 steps, 4096 observers on a 30R sphere, Minf 0.15, impulsive loading (3.36e12 evals),
+    mixed path, accuracy against the FP64 oracle on the first observers."""
+    t0 = time.time()
+    note("synthesising the coaxial stress case")
+    run = acum.Run.synth(acum.synth_params(5))
+    synth_s = time.time() - t0
+    ctx = acum.Context(0)
+    t0 = time.time()
+    run.preprocess_gpu(ctx)
+    prep_s = time.time() - t0
+    case = run.tonal_case(acum.MIXED)
+    E = sum(case.E[s] for s in range(case.nSurf))
+    evals = E * case.nTime * case.nObs
+    t0 = time.time()
+    h = ctx.stage([case])
+    stage_s = time.time() - t0
+    note("timing the mixed path")
+    ctx.run(h, acum.MIXED)
+    ms = []
+    for _ in range(max(1, args.steps)):
+        ms.append(ctx.run(h, acum.MIXED).total_ms)
+    g = ctx.fetch(h, 0, case.nTime, case.nObs)
+    ctx.release(h)
+    nob = max(1, min(args.oracle_obs, case.nObs))
+    sub = acum.TonalCase()
+    C.pointer(sub)[0] = case
+    sub.nObs = nob
+    sub.precision = acum.FP64
+    note(f"CPU oracle on {nob} observers")
+    threads = min(len(os.sched_getaffinity(0)), 16, nob)  # the oracle is parallel over observers
+    t = time.perf_counter()
+    oT, oL, oA = O.tonal_f64(sub, threads)
+    cpu_s = time.perf_counter() - t
+    gA = g[2][:, :nob]
+    d = np.abs(acum.oaspl(gA) - acum.oaspl(oA.astype(np.float32)))
+    step_ms = float(np.median(ms))
+    out = {"metric": "G src-obs-time evals/s (stress, mixed)",
+           "value": round(evals / (step_ms * 1e-3) / 1e9, 3), "unit": "G src-obs-time evals/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": "coaxial 2 x 4 blades x 50,000 elements, 2048 steps, 4096 "
+                                  "observers, Minf 0.15, impulsive loading", "evals": evals},
+           "ms_per_pass": round(step_ms, 1),
+           "accuracy": {"observers": nob, "rel_l2_pA": O.rel_l2(gA, oA),
+                        "rel_l2_pT": O.rel_l2(g[0][:, :nob], oT), "rel_l2_pL": O.rel_l2(g[1][:, :nob], oL),
+                        "max_doaspl_db": float(d.max())},
+           "cpu_baseline": {"value": round(E * case.nTime * nob / cpu_s / 1e9, 4),
+                            "unit": "G src-obs-time evals/s", "cores": threads, "kind": "port",
+                            "sample": f"FP64 oracle, first {nob} observers ({cpu_s:.1f} s)"},
+           "synth_s": round(synth_s, 1), "gpu_preprocess_call_s": round(prep_s, 1),
+           "stage_s": round(stage_s, 1)}
+    ctx.close()
+    print(json.dumps(out))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,6 +381,11 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarize-sq", metavar="DIR")
     ap.add_argument("--summarize-stats", metavar="DIR")
+    ap.add_argument("--workload", choices=("rotor", "batched", "stress"), default="rotor",
+                    help="rotor: the four-blade rotor (default, the headline line); batched: 64 "
+                         "uneven cases; stress: the coaxial 3.4e12-eval case")
+    ap.add_argument("--cases", type=int, default=64)
+    ap.add_argument("--wave", type=int, default=8)
     args = ap.parse_args()
     if args.summarize_stats:
         return summarize_stats(args.summarize_stats)
@@ -282,9 +404,10 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
-
-    def note(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    if args.workload == "batched":
+        return bench_batched(acum, O, args)
+    if args.workload == "stress":
+        return bench_stress(acum, O, args)
 
     t0 = time.time()
     note("synthesising and preprocessing the four-blade rotor")
@@ -356,7 +479,7 @@ def main():
     C.pointer(sub)[0] = case
     sub.nObs = nob
     sub.precision = acum.FP64
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    threads = min(len(os.sched_getaffinity(0)), 16, nob)  # the oracle is parallel over observers
     note(f"CPU oracle on {nob} observers")
     t = time.perf_counter()
     oT, oL, oA = O.tonal_f64(sub, threads)
