@@ -17,7 +17,8 @@ hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, i
                           float aMv, float* part);
 hipError_t launch_bpm_final(hipStream_t s, const float* part, int nchunk, int nObs, int nf2, int nT, float* spl,
                             float* oaspl);
-int bpm_occupancy_blocks();
+int bpm_occupancy_blocks(int nf2);
+int bpm_bands_per_pass(int nf2);
 void bpm_release(acum_ctx*) {}
 }  // namespace acum
 
@@ -109,8 +110,9 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   }
   hipStream_t s = ctx_stream(ctx);
   const int obs_tiles = (nObs + 63) / 64;
-  const int passes = (nf2 + 11) / 12;
-  int bpc = std::max(1, bpm_occupancy_blocks());
+  const int nb = bpm_bands_per_pass(nf2);
+  const int passes = (nf2 + nb - 1) / nb;
+  int bpc = std::max(1, bpm_occupancy_blocks(nf2));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, ctx_device(ctx)));
   const int64_t target = 4LL * bpc * prop.multiProcessorCount;

@@ -286,33 +286,48 @@ __global__ __launch_bounds__(64) void bpm_obs_kernel(const BpmSrcDev* __restrict
   }
 }
 
-// Ordered FP64 sums over chunks, time average and overall level, one thread per observer.
-__global__ void bpm_final_kernel(const float* __restrict__ part, int nchunk, int nObs, int nf2, int nT,
-                                 float* __restrict__ spl, float* __restrict__ oaspl) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= nObs) return;
+// Ordered FP64 sums over chunks, time average and overall level (bb_core.cu:565-583).  One
+// workgroup per observer, one thread per band: each thread sums its band's six components over
+// the chunks in chunk order (Kahan), then thread 0 adds the band totals in band order.
+constexpr int kFinalThreads = 64;
+
+__global__ __launch_bounds__(kFinalThreads) void bpm_final_kernel(const float* __restrict__ part,
+                                                                  int nchunk, int nObs, int nf2, int nT,
+                                                                  float* __restrict__ spl,
+                                                                  float* __restrict__ oaspl) {
+  const int o = blockIdx.x;
+  __shared__ double band_db[kFinalThreads];
   const double flo = 1.0e-10 * (1.0 + (double)nT);
   double oa = 1.0e-10;
-  for (int b = 0; b < nf2; b++) {
-    double sum[kComp], cmp[kComp];
-    for (int c = 0; c < kComp; c++) sum[c] = cmp[c] = 0.0;
-    for (int ch = 0; ch < nchunk; ch++) {
-      const float* p = part + (((int64_t)ch * nObs + o) * nf2 + b) * kComp;
-      for (int c = 0; c < kComp; c++) {
-        const double y = (double)p[c] - cmp[c], t = sum[c] + y;
-        cmp[c] = (t - sum[c]) - y;
-        sum[c] = t;
+  for (int bb = 0; bb < nf2; bb += kFinalThreads) {
+    const int b = bb + threadIdx.x;
+    if (b < nf2) {
+      double sum[kComp], cmp[kComp];
+      for (int c = 0; c < kComp; c++) sum[c] = cmp[c] = 0.0;
+      for (int ch = 0; ch < nchunk; ch++) {
+        const float* p = part + (((int64_t)ch * nObs + o) * nf2 + b) * kComp;
+        for (int c = 0; c < kComp; c++) {
+          const double y = (double)p[c] - cmp[c], t = sum[c] + y;
+          cmp[c] = (t - sum[c]) - y;
+          sum[c] = t;
+        }
       }
+      double tot = flo;
+      for (int c = 0; c < kComp; c++) tot += sum[c];
+      float* out = spl + ((int64_t)o * nf2 + b) * 7;
+      const double t0 = 10.0 * log10(tot / nT);
+      out[0] = (float)t0;
+      for (int c = 0; c < kComp; c++) out[1 + c] = (float)(10.0 * log10((flo + sum[c]) / nT));
+      band_db[threadIdx.x] = t0;
     }
-    double tot = flo;
-    for (int c = 0; c < kComp; c++) tot += sum[c];
-    float* out = spl + ((int64_t)o * nf2 + b) * 7;
-    const double t0 = 10.0 * log10(tot / nT);
-    out[0] = (float)t0;
-    for (int c = 0; c < kComp; c++) out[1 + c] = (float)(10.0 * log10((flo + sum[c]) / nT));
-    oa += exp10(0.1 * t0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int n = min(kFinalThreads, nf2 - bb);
+      for (int k = 0; k < n; k++) oa += exp10(0.1 * band_db[k]);
+    }
+    __syncthreads();
   }
-  oaspl[o] = (float)(10.0 * log10(oa));
+  if (threadIdx.x == 0) oaspl[o] = (float)(10.0 * log10(oa));
 }
 
 hipError_t launch_bpm_shape(hipStream_t s, const BpmShapeIn* in, int64_t nsrc, const float* ff, int nf2,
@@ -322,15 +337,29 @@ hipError_t launch_bpm_shape(hipStream_t s, const BpmShapeIn* in, int64_t nsrc, c
   return hipGetLastError();
 }
 
-constexpr int kBandsPerPass = 12;
+// Bands per pass of the observer kernel: the geometry of a (source, observer) pair is recomputed
+// once per pass, and each band holds six FP32 accumulators per lane; at most 17 bands a pass.
+int bpm_bands_per_pass(int nf2) {
+  const int passes = (nf2 + 16) / 17;
+  const int nb = (nf2 + passes - 1) / passes;
+  return nb <= 12 ? 12 : (nb <= 14 ? 14 : 17);
+}
 
 hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, int64_t nsrc, int64_t chunk,
                           int nchunk, int nf2, const float* obs, int nObs, float vx, float vy, float vz,
                           float aMv, float* part) {
-  for (int b0 = 0; b0 < nf2; b0 += kBandsPerPass) {
+  const int NB = bpm_bands_per_pass(nf2);
+  for (int b0 = 0; b0 < nf2; b0 += NB) {
     dim3 g((nObs + 63) / 64, nchunk);
-    hipLaunchKernelGGL(bpm_obs_kernel<kBandsPerPass>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs,
-                       nObs, vx, vy, vz, aMv, part);
+    if (NB == 17)
+      hipLaunchKernelGGL(bpm_obs_kernel<17>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
+                         vz, aMv, part);
+    else if (NB == 14)
+      hipLaunchKernelGGL(bpm_obs_kernel<14>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
+                         vz, aMv, part);
+    else
+      hipLaunchKernelGGL(bpm_obs_kernel<12>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
+                         vz, aMv, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -339,15 +368,18 @@ hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, i
 
 hipError_t launch_bpm_final(hipStream_t s, const float* part, int nchunk, int nObs, int nf2, int nT, float* spl,
                             float* oaspl) {
-  hipLaunchKernelGGL(bpm_final_kernel, dim3((nObs + 63) / 64), dim3(64), 0, s, part, nchunk, nObs, nf2, nT, spl,
-                     oaspl);
+  hipLaunchKernelGGL(bpm_final_kernel, dim3(nObs), dim3(kFinalThreads), 0, s, part, nchunk, nObs, nf2, nT,
+                     spl, oaspl);
   return hipGetLastError();
 }
 
-int bpm_occupancy_blocks() {
+int bpm_occupancy_blocks(int nf2) {
   int b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bpm_obs_kernel<kBandsPerPass>, 64, 0) != hipSuccess) return 4;
-  return b;
+  const int NB = bpm_bands_per_pass(nf2);
+  hipError_t e = NB == 17   ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bpm_obs_kernel<17>, 64, 0)
+                 : NB == 14 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bpm_obs_kernel<14>, 64, 0)
+                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, bpm_obs_kernel<12>, 64, 0);
+  return e == hipSuccess ? b : 4;
 }
 
 }  // namespace acum

@@ -217,12 +217,20 @@ def bench_bpm(acum, O, ctx, steps):
     evals = nsrc * c.nObs
     band_evals = evals * c.nf2
     g = float(np.median(gpu_ms))
+    # algorithmic FLOP (FMA = 2, transcendental = 4): 15 per band-eval (six components, clip),
+    # 90 per source-observer pair (retarded position, directivities, 1/r^2), 250 per source-band
+    # (spectral shapes)
+    flop = 15.0 * band_evals + 90.0 * evals + 250.0 * nsrc * c.nf2
     out = dict(workload=f"BPM broadband: {c.nSurf_BB} x {nsrc // c.nSurf_BB // c.nTime} sections, "
                         f"{c.nTime} steps, {c.nObs} observers, {c.nf2} bands",
                src_obs_time_evals=evals, band_evals=band_evals, gpu_ms=round(g, 3),
                call_ms=round(float(np.median(call_ms)), 2),
                value=round(evals / (g * 1e-3) / 1e9, 3), unit="G src-obs-time evals/s",
-               band_evals_per_s=round(band_evals / (g * 1e-3) / 1e9, 2))
+               band_evals_per_s=round(band_evals / (g * 1e-3) / 1e9, 2),
+               roofline={"bound": "valu", "achieved": round(flop / (g * 1e-3) / 1e12, 3),
+                         "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                         "frac": round(flop / (g * 1e-3) / PEAK_FP32, 4),
+                         "flop": flop, "kernels": "bpm_shape + bpm_obs + bpm_final"})
     # accuracy on the first observers against the FP64 oracle
     nob = 8
     ospl, ooa = O.bpm_f64(c, 0, nob)
@@ -381,9 +389,10 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarize-sq", metavar="DIR")
     ap.add_argument("--summarize-stats", metavar="DIR")
-    ap.add_argument("--workload", choices=("rotor", "batched", "stress"), default="rotor",
+    ap.add_argument("--workload", choices=("rotor", "batched", "stress", "bpm"), default="rotor",
                     help="rotor: the four-blade rotor (default, the headline line); batched: 64 "
-                         "uneven cases; stress: the coaxial 3.4e12-eval case")
+                         "uneven cases; stress: the coaxial 3.4e12-eval case; bpm: the "
+                         "broadband leg alone")
     ap.add_argument("--cases", type=int, default=64)
     ap.add_argument("--wave", type=int, default=8)
     args = ap.parse_args()
@@ -408,6 +417,12 @@ def main():
         return bench_batched(acum, O, args)
     if args.workload == "stress":
         return bench_stress(acum, O, args)
+    if args.workload == "bpm":
+        ctx = acum.Context(0)
+        out = bench_bpm(acum, O, ctx, args.steps)
+        ctx.close()
+        print(json.dumps(out))
+        return 0
 
     t0 = time.time()
     note("synthesising and preprocessing the four-blade rotor")
