@@ -5,6 +5,7 @@
 // would (correctly rounded), errors are returned instead of exit(1), offsets are 64-bit.
 // An optional BB_inputs/BB_bands.dat ("n" then n frequencies) replaces the reference's fixed
 // 42-band table; the reference ignores that file, so the directory stays readable by both.
+#include <algorithm>
 #include <cctype>
 #include <cerrno>
 #include <cmath>
@@ -14,6 +15,7 @@
 #include <string>
 #include <sys/stat.h>
 #include <vector>
+#include <omp.h>
 #include "run.hpp"
 
 namespace acum_host {
@@ -126,6 +128,128 @@ struct Text {
   }
 };
 
+// ---- parallel number parsing for the large surface files -----------------------------------
+// A decimal token (sign, digits, point, digits, exponent) with at most 19 significant digits and
+// a power of ten within 1e+-22 is m * 10^e with m and 10^e exact doubles, so one IEEE multiply or
+// divide gives the correctly rounded double (Clinger's fast path); rounding that double to float
+// is the correctly rounded float unless the double sits exactly on a float rounding midpoint
+// (the only case where the second rounding can differ).  Every other token goes to strtof, the
+// function fscanf("%f") uses, so results are those of the reference's reader bit for bit.
+inline bool is_space(char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\f' || c == '\v'; }
+
+bool fast_float(const char* p, const char* end, float* out) {
+  static const double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                                    1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                                    1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+  bool neg = false;
+  if (p < end && (*p == '-' || *p == '+')) neg = *p++ == '-';
+  uint64_t m = 0;
+  int sig = 0, e10 = 0;
+  bool any = false;
+  for (; p < end && *p >= '0' && *p <= '9'; p++) {
+    any = true;
+    if (m == 0 && *p == '0') continue;
+    if (sig == 19) return false;
+    m = m * 10 + (uint64_t)(*p - '0');
+    sig++;
+  }
+  if (p < end && *p == '.') {
+    for (p++; p < end && *p >= '0' && *p <= '9'; p++) {
+      any = true;
+      if (m == 0 && *p == '0') {
+        e10--;
+        continue;
+      }
+      if (sig == 19) return false;
+      m = m * 10 + (uint64_t)(*p - '0');
+      sig++;
+      e10--;
+    }
+  }
+  if (!any) return false;
+  if (p < end && (*p == 'e' || *p == 'E')) {
+    p++;
+    bool eneg = false;
+    if (p < end && (*p == '-' || *p == '+')) eneg = *p++ == '-';
+    if (!(p < end && *p >= '0' && *p <= '9')) return false;
+    int x = 0;
+    for (; p < end && *p >= '0' && *p <= '9'; p++)
+      if (x < 10000) x = x * 10 + (*p - '0');
+    e10 += eneg ? -x : x;
+  }
+  if (p != end) return false;
+  if (m == 0) {
+    *out = neg ? -0.0f : 0.0f;
+    return true;
+  }
+  if (m > (1ULL << 53) || e10 < -22 || e10 > 22) return false;
+  double d = (double)m;
+  d = e10 < 0 ? d / kPow10[-e10] : d * kPow10[e10];
+  if (!(d >= 1.1754943508222875e-38 && d <= 3.4028234663852886e+38)) return false;  // float normal range
+  uint64_t bits;
+  std::memcpy(&bits, &d, sizeof(bits));
+  if ((bits & ((1ULL << 29) - 1)) == (1ULL << 28)) return false;  // exactly on a float midpoint
+  *out = (float)(neg ? -d : d);
+  return true;
+}
+
+// Parses the first `count` whitespace-separated numbers of [b, e) (e points at a NUL) in parallel
+// and hands number i to store(i, value).  Returns -1 on success, or the index of the first number
+// that is missing or not a number.
+template <typename Store>
+int64_t parse_numbers(const char* b, const char* e, int64_t count, Store store) {
+  const int T = std::max(1, omp_get_max_threads());
+  const int64_t len = e - b;
+  std::vector<const char*> cut(T + 1);
+  cut[0] = b;
+  cut[T] = e;
+  for (int t = 1; t < T; t++) {  // chunk boundaries at whitespace
+    const char* c = b + len * t / T;
+    if (c < cut[t - 1]) c = cut[t - 1];
+    while (c < e && !is_space(*c)) c++;
+    cut[t] = c;
+  }
+  std::vector<int64_t> ntok(T + 1, 0);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+  for (int t = 0; t < T; t++) {
+    int64_t n = 0;
+    bool in = false;
+    for (const char* c = cut[t]; c < cut[t + 1]; c++) {
+      const bool sp = is_space(*c);
+      if (!sp && !in) n++;
+      in = !sp;
+    }
+    ntok[t + 1] = n;
+  }
+  for (int t = 0; t < T; t++) ntok[t + 1] += ntok[t];
+  int64_t bad = ntok[T] < count ? ntok[T] : -1;
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+  for (int t = 0; t < T; t++) {
+    int64_t i = ntok[t];
+    const char* c = cut[t];
+    const char* stop = cut[t + 1];
+    while (i < count) {
+      while (c < stop && is_space(*c)) c++;
+      if (c >= stop) break;
+      const char* s0 = c;
+      while (c < stop && !is_space(*c)) c++;
+      float v;
+      if (!fast_float(s0, c, &v)) {
+        char* endp = nullptr;
+        v = std::strtof(s0, &endp);
+        if (endp != c) {
+#pragma omp critical
+          if (bad < 0 || i < bad) bad = i;
+          break;
+        }
+      }
+      store(i, v);
+      i++;
+    }
+  }
+  return bad;
+}
+
 std::string join(const std::string& dir, const std::string& name) {
   if (dir.empty()) return name;
   return dir.back() == '/' ? dir + name : dir + "/" + name;
@@ -236,20 +360,31 @@ int read_surf(const std::string& dir, acum_run* r) {
     r->E[i] = e;
   }
   layout_surfaces(r);
+  // The body of surface i is nT blocks of E lines "x y z CP A nx ny nz" (time-major); number n of
+  // the body belongs to step n / (8 E), element (n / 8) % E, field n % 8.
+  static const int kDst[8] = {0, 1, 2, 8, 3, 4, 5, 6};  // X[0..3] = 0..3, N[0..2] = 4..6, PR[0] = 8
   for (int64_t i = 0; i < nS; i++) {
     Text& t = files[i];
-    for (int64_t k = 0; k < nT; k++) {
-      for (int64_t j = 0; j < r->E[i]; j++) {
-        float* Xh = &r->X[r->XPtr[i] + (j * nT + k) * 4];
-        float* Nh = &r->N[r->NPtr[i] + (j * nT + k) * 6];
-        float* Ph = &r->PR[r->PRPtr[i] + (j * nT + k) * 2];
-        if (!t.f32(&Xh[0]) || !t.f32(&Xh[1]) || !t.f32(&Xh[2]) || !t.f32(&Ph[0]) ||
-            !t.f32(&Xh[3]) || !t.f32(&Nh[0]) || !t.f32(&Nh[1]) || !t.f32(&Nh[2]))
-          return fail(ACUM_ERR_IO, "truncated surface file surface_" + std::to_string(i + 1) +
-                                       ".dat at step " + std::to_string(k) + " element " +
-                                       std::to_string(j));
-      }
-    }
+    const int64_t E = r->E[i];
+    float* X = r->X.data() + r->XPtr[i];
+    float* N = r->N.data() + r->NPtr[i];
+    float* P = r->PR.data() + r->PRPtr[i];
+    const char* b = t.buf.data() + t.pos;
+    const char* e = t.buf.data() + t.buf.size() - 1;
+    const int64_t bad = parse_numbers(b, e, nT * E * 8, [&](int64_t n, float v) {
+      const int64_t k = n / (8 * E), j = (n / 8) % E;
+      const int f = kDst[n % 8];
+      const int64_t q = j * nT + k;
+      if (f < 4) X[q * 4 + f] = v;
+      else if (f < 8) N[q * 6 + (f - 4)] = v;
+      else P[q * 2] = v;
+    });
+    if (bad >= 0)
+      return fail(ACUM_ERR_IO, "truncated or malformed surface file surface_" + std::to_string(i + 1) +
+                                   ".dat at step " + std::to_string(bad / (8 * E)) + " element " +
+                                   std::to_string((bad / 8) % E));
+    t.buf.clear();
+    t.buf.shrink_to_fit();
   }
   return ACUM_OK;
 }

@@ -286,3 +286,39 @@ def test_broadband_reader_matches_reference(tmp_path):
     assert np.array_equal(g(c.bbdata_t, int((nst * nm_ * na * nr * 7).sum())), ref("bb_bbdata_t"))
     assert np.array_equal(g(c.ff, c.nf2), ref("bb_ff"))
     assert [nS, c.t_del, c.nf2] == list(ref("bb_ints"))
+
+
+def test_surface_reader_matches_strtof(tmp_path):
+    """The parallel surface reader (exact fast path + strtof fallback) gives strtof's float for
+    every token spelling, including values on and next to float rounding midpoints."""
+    run = acum.Run.synth(acum.synth_params(1, nChord=3, nSpan=2, nTime=32))
+    d = tmp_path / "r"
+    run.save(str(d))
+    libc = C.CDLL(None)
+    libc.strtof.restype = C.c_float
+    libc.strtof.argtypes = [C.c_char_p, C.POINTER(C.c_char_p)]
+    rng = np.random.default_rng(17)
+    E = run.tonal_case_raw_sizes()[0][0]
+    nT = 32
+    n = nT * E * 8
+    vals = rng.standard_normal(n) * 10.0 ** rng.integers(-12, 12, n)
+    # exact float midpoints and their neighbours, printed with 17 significant digits
+    f = rng.standard_normal(n // 4).astype(np.float32)
+    nxt = np.nextafter(f, np.float32(np.inf))
+    mid = (f.astype(np.float64) + nxt.astype(np.float64)) / 2
+    vals[: n // 4] = mid
+    vals[n // 4: n // 2] = np.nextafter(mid, np.inf)
+    fmts = ["%.9e", "%.17g", "%.6f", "%g", "%+.8E", "%.3e", "%.25f", "%d"]
+    toks = []
+    for i, v in enumerate(vals):
+        fm = fmts[i % len(fmts)]
+        toks.append(fm % (int(v) if fm == "%d" else v))
+    toks[5] = "1e-40"      # subnormal
+    toks[13] = "-0.0"
+    toks[21] = "0.000000000000000000000000001234567"
+    toks[29] = "3.14159265358979323846264338327950288419716939937510"
+    lines = [f"{E} {nT}"] + [" ".join(toks[i:i + 8]) for i in range(0, n, 8)]
+    (d / "surfaces" / "surface_1.dat").write_text("\n".join(lines) + "\n")
+    got = acum.Run.load(str(d)).surface_raw(0).reshape(-1)
+    want = np.array([libc.strtof(t.encode(), None) for t in toks], np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
