@@ -7,6 +7,7 @@
 // 42-band table; the reference ignores that file, so the directory stays readable by both.
 #include <algorithm>
 #include <cctype>
+#include <charconv>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -581,19 +582,59 @@ int read_bb(const std::string& dir, acum_run* r) {
   return ACUM_OK;
 }
 
+// Appends v as printf's "%<width>.<prec>E" / "%<width>.<prec>e" would write it.  std::to_chars
+// in scientific form with a precision gives the same correctly rounded digits as printf; the
+// exponent letter and the width are adjusted here; NaN and infinity go through snprintf.
+void append_sci(std::string& s, double v, int prec, bool upper, int width) {
+  char buf[48];
+  int n;
+  if (std::isfinite(v)) {
+    const auto r = std::to_chars(buf, buf + sizeof(buf), v, std::chars_format::scientific, prec);
+    n = (int)(r.ptr - buf);
+    if (upper)
+      for (int i = 0; i < n; i++)
+        if (buf[i] == 'e') buf[i] = 'E';
+  } else {
+    n = std::snprintf(buf, sizeof(buf), upper ? "%.*E" : "%.*e", prec, v);
+  }
+  for (int i = n; i < width; i++) s.push_back(' ');
+  s.append(buf, (size_t)n);
+}
+
 int write_time_history(const acum_run* r, const std::string& dir, const char* name, const float* p) {
   const std::string path = join(dir, name);
   FILE* f = std::fopen(path.c_str(), "w");
   if (!f) return fail(ACUM_ERR_IO, "cannot write " + path);
   std::fprintf(f, "# ntime nobs\n");
   std::fprintf(f, "%d %d\n", (int)r->nTime, (int)r->nObs);
-  std::vector<char> line;
-  for (int64_t i = 0; i < r->nTime; i++) {
-    std::fprintf(f, " %14.8E", (double)((float)i * r->dPsi_f));
-    for (int64_t j = 0; j < r->nObs; j++) std::fprintf(f, " %14.8E", (double)p[i * r->nObs + j]);
-    std::fprintf(f, "\n");
+  // rows " %14.8E" (t dPsi, radians) then nObs x " %14.8E" (io.cu:143-157), formatted in parallel
+  const int64_t nT = r->nTime, nObs = r->nObs;
+  const int64_t block = 64;
+  std::vector<std::string> rows((size_t)block);
+  for (int64_t i0 = 0; i0 < nT; i0 += block) {
+    const int64_t i1 = std::min(nT, i0 + block);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t i = i0; i < i1; i++) {
+      std::string& s = rows[(size_t)(i - i0)];
+      s.clear();
+      s.reserve((size_t)(nObs + 1) * 16 + 1);
+      s.push_back(' ');
+      append_sci(s, (double)((float)i * r->dPsi_f), 8, true, 14);
+      for (int64_t j = 0; j < nObs; j++) {
+        s.push_back(' ');
+        append_sci(s, (double)p[i * nObs + j], 8, true, 14);
+      }
+      s.push_back('\n');
+    }
+    for (int64_t i = i0; i < i1; i++) {
+      const std::string& s = rows[(size_t)(i - i0)];
+      if (std::fwrite(s.data(), 1, s.size(), f) != s.size()) {
+        std::fclose(f);
+        return fail(ACUM_ERR_IO, "write failed: " + path);
+      }
+    }
   }
-  std::fclose(f);
+  if (std::fclose(f) != 0) return fail(ACUM_ERR_IO, "write failed: " + path);
   return ACUM_OK;
 }
 
@@ -724,12 +765,20 @@ int acum_run_write_bb(const acum_run* r, const char* dir, const float* BBoaspl, 
   static const int comp[7] = {0, 3, 2, 1, 5, 4, 6};  // io.cu:737-792
   for (int z = 0; z < 7; z++) {
     std::fprintf(f, " ZONE T=\"%s\"\n", names[z]);
-    for (int j = 0; j < nf2; j++) {
-      std::fprintf(f, "   %e", (double)r->ff[j]);
-      for (int64_t i = 0; i < r->nObs; i++)
-        std::fprintf(f, "  %e", (double)BBspl[i * nf2 * 7 + j * 7 + comp[z]]);
-      std::fprintf(f, "\n");
+    std::vector<std::string> rows((size_t)nf2);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 0; j < nf2; j++) {  // "   %e" then nObs x "  %e" (io.cu:738-791)
+      std::string& s = rows[(size_t)j];
+      s.reserve((size_t)(r->nObs + 1) * 16 + 1);
+      s.append("   ");
+      append_sci(s, (double)r->ff[j], 6, false, 0);
+      for (int64_t i = 0; i < r->nObs; i++) {
+        s.append("  ");
+        append_sci(s, (double)BBspl[i * nf2 * 7 + j * 7 + comp[z]], 6, false, 0);
+      }
+      s.push_back('\n');
     }
+    for (const std::string& s : rows) std::fwrite(s.data(), 1, s.size(), f);
   }
   std::fclose(f);
   return ACUM_OK;

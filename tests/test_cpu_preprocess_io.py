@@ -322,3 +322,34 @@ def test_surface_reader_matches_strtof(tmp_path):
     got = acum.Run.load(str(d)).surface_raw(0).reshape(-1)
     want = np.array([libc.strtof(t.encode(), None) for t in toks], np.float32)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_tonal_writer_matches_printf(tmp_path):
+    """pT/pL/pA.out rows are " %14.8E" of the float values as io.cu:143-192 prints them,
+    checked against C's snprintf over random floats across the whole range and special values."""
+    run = acum.Run.synth(acum.synth_params(1, nChord=2, nSpan=2, nTime=24, nObs=37))
+    inp = run.inputs()
+    c = run.preprocess().tonal_case(acum.FP64)
+    nT, nObs = c.nTime, c.nObs
+    rng = np.random.default_rng(3)
+    bits = rng.integers(0, 2 ** 32, nT * nObs, dtype=np.uint64).astype(np.uint32)
+    vals = bits.view(np.float32).copy()
+    vals[:8] = [0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -3.4028235e38, 1.0]
+    pT = vals.copy()
+    pL = (vals * 0.5).astype(np.float32)
+    pA = np.arange(nT * nObs, dtype=np.float32) * np.float32(1.37e-3)
+    run.write_tonal(str(tmp_path), pT, pL, pA)
+    libc = C.CDLL(None)
+    buf = C.create_string_buffer(64)
+
+    def pf(v):
+        libc.snprintf(buf, 64, b" %14.8E", C.c_double(float(v)))
+        return buf.value.decode()
+
+    dpsi = np.float32(inp.dPsi)
+    for name, arr in (("pT.out", pT), ("pL.out", pL), ("pA.out", pA)):
+        lines = [f"# ntime nobs", f"{nT} {nObs}"]
+        for i in range(nT):
+            row = pf(np.float32(i) * dpsi) + "".join(pf(arr[i * nObs + j]) for j in range(nObs))
+            lines.append(row)
+        assert (tmp_path / name).read_text() == "\n".join(lines) + "\n", name
