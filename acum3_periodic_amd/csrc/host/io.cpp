@@ -14,7 +14,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <fcntl.h>
+#include <sys/mman.h>
 #include <sys/stat.h>
+#include <unistd.h>
 #include <vector>
 #include <omp.h>
 #include "run.hpp"
@@ -66,10 +69,16 @@ void layout_surfaces(acum_run* r) {
     }
     etot += r->E[i];
   }
-  r->X.assign((size_t)(etot * nT * 4), 0.0f);
-  r->N.assign((size_t)(etot * nT * 6), 0.0f);
-  r->V.assign((size_t)(etot * nT * 6), 0.0f);
-  r->PR.assign((size_t)(etot * nT * 2), 0.0f);
+  fvec* arr[4] = {&r->X, &r->N, &r->V, &r->PR};
+  const int64_t width[4] = {4, 6, 6, 2};
+  for (int a = 0; a < 4; a++) {
+    arr[a]->clear();
+    arr[a]->resize((size_t)(etot * nT * width[a]));  // uninitialised
+    float* p = arr[a]->data();
+    const int64_t n = (int64_t)arr[a]->size();
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) p[i] = 0.0f;
+  }
 }
 
 }  // namespace acum_host
@@ -236,9 +245,16 @@ int64_t parse_numbers(const char* b, const char* e, int64_t count, Store store) 
       while (c < stop && !is_space(*c)) c++;
       float v;
       if (!fast_float(s0, c, &v)) {
+        // strtof on a NUL-terminated copy (the text may be a file mapping without a NUL)
+        char tok[128];
+        const size_t tl = (size_t)(c - s0);
         char* endp = nullptr;
-        v = std::strtof(s0, &endp);
-        if (endp != c) {
+        if (tl < sizeof(tok)) {
+          std::memcpy(tok, s0, tl);
+          tok[tl] = '\0';
+          v = std::strtof(tok, &endp);
+        }
+        if (tl >= sizeof(tok) || endp != tok + tl) {
 #pragma omp critical
           if (bad < 0 || i < bad) bad = i;
           break;
@@ -250,6 +266,36 @@ int64_t parse_numbers(const char* b, const char* e, int64_t count, Store store) 
   }
   return bad;
 }
+
+// Read-only mapping of a whole file.
+struct MappedFile {
+  const char* data = nullptr;
+  size_t size = 0;
+  bool open(const std::string& path) {
+    const int fd = ::open(path.c_str(), O_RDONLY);
+    if (fd < 0) return false;
+    struct stat st;
+    if (::fstat(fd, &st) != 0) {
+      ::close(fd);
+      return false;
+    }
+    size = (size_t)st.st_size;
+    if (size > 0) {
+      void* p = ::mmap(nullptr, size, PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
+      if (p == MAP_FAILED) {
+        ::close(fd);
+        return false;
+      }
+      ::madvise(p, size, MADV_SEQUENTIAL);
+      data = (const char*)p;
+    }
+    ::close(fd);
+    return true;
+  }
+  ~MappedFile() {
+    if (data) ::munmap((void*)data, size);
+  }
+};
 
 std::string join(const std::string& dir, const std::string& name) {
   if (dir.empty()) return name;
@@ -345,47 +391,75 @@ int read_observers(const std::string& path, acum_run* r) {
 }
 
 // read_surf, io.cu:206-333 (impermeable branch): nTime blocks of E lines x y z CP A nx ny nz.
+// Each file is mapped, its numbers parsed in parallel into a time-major buffer, which is then
+// transposed in tiles into the element-major arrays.
 int read_surf(const std::string& dir, acum_run* r) {
   const int64_t nS = r->in.nSurf, nT = r->in.nTime;
   r->nSurf = nS;
   r->nTime = nT;
   r->E.assign(nS, 0);
-  std::vector<Text> files(nS);
+  std::vector<MappedFile> files(nS);
+  std::vector<size_t> body(nS, 0);
   for (int64_t i = 0; i < nS; i++) {
-    std::string p = join(dir, "surfaces/surface_" + std::to_string(i + 1) + ".dat");
-    if (!files[i].load(p)) return fail(ACUM_ERR_IO, "missing surface file " + p);
-    int32_t e = 0, nt = 0;
-    if (!files[i].i32(&e) || !files[i].i32(&nt) || e < 1)
-      return fail(ACUM_ERR_IO, "bad header in " + p);
+    const std::string p = join(dir, "surfaces/surface_" + std::to_string(i + 1) + ".dat");
+    if (!files[i].open(p)) return fail(ACUM_ERR_IO, "missing surface file " + p);
+    // header: the first two integers "E nt" (fscanf "%d %d\n", io.cu:235)
+    const char* d = files[i].data;
+    const size_t n = files[i].size;
+    size_t q = 0;
+    int64_t hv[2] = {0, 0};
+    for (int h = 0; h < 2; h++) {
+      while (q < n && is_space(d[q])) q++;
+      const size_t t0 = q;
+      while (q < n && !is_space(d[q])) q++;
+      const std::string tok(d + t0, q - t0);
+      char* endp = nullptr;
+      hv[h] = std::strtoll(tok.c_str(), &endp, 10);
+      if (tok.empty() || *endp != '\0') return fail(ACUM_ERR_IO, "bad header in " + p);
+    }
+    const int64_t e = hv[0], nt = hv[1];
+    if (e < 1) return fail(ACUM_ERR_IO, "bad header in " + p);
     if (nt != nT) std::fprintf(stderr, "\tInconsistent number of steps in %s\n", p.c_str());
     r->E[i] = e;
+    body[i] = q;
   }
   layout_surfaces(r);
-  // The body of surface i is nT blocks of E lines "x y z CP A nx ny nz" (time-major); number n of
-  // the body belongs to step n / (8 E), element (n / 8) % E, field n % 8.
-  static const int kDst[8] = {0, 1, 2, 8, 3, 4, 5, 6};  // X[0..3] = 0..3, N[0..2] = 4..6, PR[0] = 8
+  int64_t Emax = 0;
+  for (int64_t i = 0; i < nS; i++) Emax = std::max(Emax, r->E[i]);
+  fvec tmp((size_t)(nT * Emax * 8));  // time-major numbers of one file, first touched by the parse
   for (int64_t i = 0; i < nS; i++) {
-    Text& t = files[i];
     const int64_t E = r->E[i];
-    float* X = r->X.data() + r->XPtr[i];
-    float* N = r->N.data() + r->NPtr[i];
-    float* P = r->PR.data() + r->PRPtr[i];
-    const char* b = t.buf.data() + t.pos;
-    const char* e = t.buf.data() + t.buf.size() - 1;
-    const int64_t bad = parse_numbers(b, e, nT * E * 8, [&](int64_t n, float v) {
-      const int64_t k = n / (8 * E), j = (n / 8) % E;
-      const int f = kDst[n % 8];
-      const int64_t q = j * nT + k;
-      if (f < 4) X[q * 4 + f] = v;
-      else if (f < 8) N[q * 6 + (f - 4)] = v;
-      else P[q * 2] = v;
-    });
+    const int64_t count = nT * E * 8;
+    const char* b = files[i].data + body[i];
+    const char* e = files[i].data + files[i].size;
+    const int64_t bad = parse_numbers(b, e, count, [&](int64_t n, float v) { tmp[(size_t)n] = v; });
     if (bad >= 0)
       return fail(ACUM_ERR_IO, "truncated or malformed surface file surface_" + std::to_string(i + 1) +
                                    ".dat at step " + std::to_string(bad / (8 * E)) + " element " +
                                    std::to_string((bad / 8) % E));
-    t.buf.clear();
-    t.buf.shrink_to_fit();
+    // tmp[k][j][x y z CP A nx ny nz] -> X[j][k][x y z A], N[j][k][n], PR[j][k][CP]
+    float* X = r->X.data() + r->XPtr[i];
+    float* N = r->N.data() + r->NPtr[i];
+    float* P = r->PR.data() + r->PRPtr[i];
+    const int64_t TB = 64;
+#pragma omp parallel for schedule(dynamic, 1) collapse(2)
+    for (int64_t j0 = 0; j0 < E; j0 += TB)
+      for (int64_t k0 = 0; k0 < nT; k0 += TB) {
+        const int64_t j1 = std::min(E, j0 + TB), k1 = std::min(nT, k0 + TB);
+        for (int64_t j = j0; j < j1; j++)
+          for (int64_t k = k0; k < k1; k++) {
+            const float* t = &tmp[(size_t)((k * E + j) * 8)];
+            const int64_t q = j * nT + k;
+            X[q * 4 + 0] = t[0];
+            X[q * 4 + 1] = t[1];
+            X[q * 4 + 2] = t[2];
+            X[q * 4 + 3] = t[4];
+            P[q * 2] = t[3];
+            N[q * 6 + 0] = t[5];
+            N[q * 6 + 1] = t[6];
+            N[q * 6 + 2] = t[7];
+          }
+      }
   }
   return ACUM_OK;
 }

@@ -2,9 +2,33 @@
 // variables.cuh:1-60) gathered into one object, with 64-bit offsets.
 #pragma once
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <string>
 #include <vector>
 #include "../../../include/acum.h"
+
+// Allocator that leaves new elements uninitialised, so the large element arrays can be filled
+// (first-touched) in parallel instead of being zeroed by one thread.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+  template <typename U>
+  struct rebind {
+    using other = NoInitAlloc<U>;
+  };
+  NoInitAlloc() = default;
+  template <typename U>
+  NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+  template <typename U>
+  void construct(U* p) noexcept {
+    ::new ((void*)p) U;
+  }
+  template <typename U, typename... A>
+  void construct(U* p, A&&... a) {
+    ::new ((void*)p) U(std::forward<A>(a)...);
+  }
+};
+using fvec = std::vector<float, NoInitAlloc<float>>;
 
 struct acum_run {
   acum_inputs in{};
@@ -15,7 +39,7 @@ struct acum_run {
   // tonal surfaces, reference AoS layout (io.cu:206-333)
   int64_t nSurf = 0, nTime = 0;
   std::vector<int64_t> E, XPtr, NPtr, VPtr, PRPtr;
-  std::vector<float> X, N, V, PR;
+  fvec X, N, V, PR;  // filled in parallel (layout_surfaces zeroes them with all threads)
   bool preprocessed = false;
   // broadband (io.cu:337-700)
   bool has_bb = false;

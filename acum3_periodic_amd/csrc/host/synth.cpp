@@ -18,6 +18,7 @@
 // Observers: arc in the x-z plane, hemisphere below the disk (nAz x nEl) or Fibonacci sphere.
 // Scalars are rounded to float, as reading them from the files would do.
 #include <algorithm>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -527,14 +528,32 @@ int acum_run_save(const acum_run* r, const char* dir) {
     f = std::fopen(path("surfaces/surface_" + std::to_string(s + 1) + ".dat").c_str(), "w");
     if (!f) return fail(ACUM_ERR_IO, "cannot write surface file");
     std::fprintf(f, "%d %d\n", (int)r->E[s], (int)nT);
-    for (int64_t k = 0; k < nT; k++)
-      for (int64_t j = 0; j < r->E[s]; j++) {
-        const float* Xh = &r->X[r->XPtr[s] + (j * nT + k) * 4];
-        const float* Nh = &r->N[r->NPtr[s] + (j * nT + k) * 6];
-        const float* Ph = &r->PR[r->PRPtr[s] + (j * nT + k) * 2];
-        std::fprintf(f, "%.9g %.9g %.9g %.9g %.9g %.9g %.9g %.9g\n", Xh[0], Xh[1], Xh[2], Ph[0], Xh[3],
-                     Nh[0], Nh[1], Nh[2]);
+    // one block of E lines per step, formatted in parallel; each value is the shortest decimal
+    // that reads back as the same float
+    const int64_t E = r->E[s];
+    std::vector<std::string> blocks(64);
+    for (int64_t k0 = 0; k0 < nT; k0 += 64) {
+      const int64_t k1 = std::min<int64_t>(nT, k0 + 64);
+#pragma omp parallel for schedule(dynamic, 1)
+      for (int64_t k = k0; k < k1; k++) {
+        std::string& b = blocks[(size_t)(k - k0)];
+        b.clear();
+        b.reserve((size_t)E * 8 * 16);
+        char num[32];
+        for (int64_t j = 0; j < E; j++) {
+          const float* Xh = &r->X[r->XPtr[s] + (j * nT + k) * 4];
+          const float* Nh = &r->N[r->NPtr[s] + (j * nT + k) * 6];
+          const float* Ph = &r->PR[r->PRPtr[s] + (j * nT + k) * 2];
+          const float v[8] = {Xh[0], Xh[1], Xh[2], Ph[0], Xh[3], Nh[0], Nh[1], Nh[2]};
+          for (int q = 0; q < 8; q++) {
+            const auto res = std::to_chars(num, num + sizeof(num), v[q]);
+            b.append(num, (size_t)(res.ptr - num));
+            b.push_back(q == 7 ? '\n' : ' ');
+          }
+        }
       }
+      for (int64_t k = k0; k < k1; k++) std::fwrite(blocks[(size_t)(k - k0)].data(), 1, blocks[(size_t)(k - k0)].size(), f);
+    }
     std::fclose(f);
   }
   if (r->has_bb) {
