@@ -23,7 +23,7 @@ hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* ca
 hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, int nT, double SB, void* pos);
 hipError_t launch_tonal_attr(bool mixed, hipStream_t s, const float* X, const float* N,
                              const float* V, const float* PR, int64_t n, double a0, double rhoRef,
-                             void* attr);
+                             void* attr, uint32_t* flags);
 hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes);
 hipError_t launch_reset(hipStream_t s, int* counter, uint32_t* flags);
 hipError_t launch_tonal_bins(hipStream_t s, const void* pos, int64_t nE, const TimeConst& k,
@@ -234,6 +234,7 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
   if (!sc.d_aos[0]) return ctx_fail(ctx, ACUM_ERR_ARG, "staged element data already released");
   const bool mixed = prec == ACUM_MIXED;
   const int64_t n = sc.nE * sc.nT;
+  HIPCHK(ctx, launch_reset(ctx->stream, ctx->d_counter, ctx->d_flags));
   if (!sc.d_pos) {
     HIPCHK(ctx, hipMalloc(&sc.d_pos, (size_t)n * sizeof(PosRec)));
     HIPCHK(ctx, launch_tonal_pos(ctx->stream, sc.d_aos[0], n, (int)sc.nT, sc.k.SB, sc.d_pos));
@@ -241,7 +242,17 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
   const size_t bytes = (size_t)n * (mixed ? sizeof(AttrT<float>) : sizeof(AttrT<double>));
   HIPCHK(ctx, hipMalloc(&sc.d_attr[prec], bytes));
   HIPCHK(ctx, launch_tonal_attr(mixed, ctx->stream, sc.d_aos[0], sc.d_aos[1], sc.d_aos[2], sc.d_aos[3],
-                                n, sc.k.a0, sc.rhoRef, sc.d_attr[prec]));
+                                n, sc.k.a0, sc.rhoRef, sc.d_attr[prec], ctx->d_flags));
+  {
+    uint32_t f = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&f, ctx->d_flags, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (f & kFlagNumeric) {
+      (void)hipFree(sc.d_attr[prec]);
+      sc.d_attr[prec] = nullptr;
+      return ctx_fail(ctx, ACUM_ERR_NUMERIC, "non-finite value in the element data");
+    }
+  }
   if (sc.d_attr[0] && sc.d_attr[1]) {  // both record sets built: the AoS copy is no longer needed
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     for (auto& p : sc.d_aos) {

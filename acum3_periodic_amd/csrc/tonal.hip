@@ -456,7 +456,9 @@ __global__ void tonal_pos_kernel(const float* __restrict__ X, int64_t n, int nT,
 template <typename R>
 __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __restrict__ N,
                                   const float* __restrict__ V, const float* __restrict__ PR,
-                                  int64_t n, double a0, double rhoRef, AttrT<R>* __restrict__ attr) {
+                                  int64_t n, double a0, double rhoRef, AttrT<R>* __restrict__ attr,
+                                  uint32_t* __restrict__ flags) {
+  bool bad = false;
   const double inv_a0 = 1.0 / a0;
   const double D1 = 1.0 / (4.0 * M_PI * a0);
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
@@ -465,6 +467,9 @@ __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __re
     const float* nn = N + 6 * q;
     const float* v = V + 6 * q;
     const double P = PR[2 * q], dP = PR[2 * q + 1], ds = x[3];
+    bad |= !isfinite(P) || !isfinite(dP);
+    for (int i = 0; i < 6; i++) bad |= !isfinite(nn[i]) || !isfinite(v[i]);
+    for (int i = 0; i < 4; i++) bad |= !isfinite(x[i]);
     AttrT<R> o;
     double Vn = 0.0, VV = 0.0;
     for (int i = 0; i < 3; i++) {
@@ -482,6 +487,7 @@ __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __re
     o.cL = (R)(ds * D1);
     attr[q] = o;
   }
+  if (bad) atomicOr(flags, kFlagNumeric);  // non-finite element input
 }
 
 template <typename T>
@@ -605,14 +611,14 @@ hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, int nT, do
 
 hipError_t launch_tonal_attr(bool mixed, hipStream_t s, const float* X, const float* N,
                              const float* V, const float* PR, int64_t n, double a0, double rhoRef,
-                             void* attr) {
+                             void* attr, uint32_t* flags) {
   const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
   if (mixed)
     hipLaunchKernelGGL(tonal_attr_kernel<float>, dim3(grid), dim3(256), 0, s, X, N, V, PR, n, a0,
-                       rhoRef, (AttrT<float>*)attr);
+                       rhoRef, (AttrT<float>*)attr, flags);
   else
     hipLaunchKernelGGL(tonal_attr_kernel<double>, dim3(grid), dim3(256), 0, s, X, N, V, PR, n, a0,
-                       rhoRef, (AttrT<double>*)attr);
+                       rhoRef, (AttrT<double>*)attr, flags);
   return hipGetLastError();
 }
 

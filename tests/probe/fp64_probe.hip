@@ -14,6 +14,25 @@ __global__ void probe_kernel(const double* __restrict__ x, double* __restrict__ 
   }
 }
 
+// XCC_ID hardware register (bits 3:0) seen by each workgroup, the value the eval kernel uses to
+// pick its work queue.
+__global__ void xcc_kernel(int* __restrict__ out, int n) {
+  if (threadIdx.x == 0 && (int)blockIdx.x < n)
+    out[blockIdx.x] = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & 15);
+}
+
+extern "C" int probe_xcc(int* ids, int nblocks) {
+  int* d = nullptr;
+  hipError_t e = hipMalloc(&d, nblocks * sizeof(int));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(xcc_kernel, dim3(nblocks), dim3(64), 0, 0, d, nblocks);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(ids, d, nblocks * sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return (int)e;
+}
+
 // x: host array of n positive doubles; s, r: host outputs (sqrt, reciprocal).  Returns 0 on
 // success, a HIP error code otherwise.
 extern "C" int probe_fp64(const double* x, double* s, double* r, int64_t n) {

@@ -1,5 +1,6 @@
-"""The FP64 helpers of the tonal kernels on the device: sqrt_pos must be correctly rounded (the
-canonical observer time, and so every bin index, depends on it) and rcp_nr within 1 ulp."""
+"""Device helpers of the tonal kernels: sqrt_pos must be correctly rounded (the canonical observer
+time, and so every bin index, depends on it), rcp_nr within 1 ulp, and the XCC_ID register the
+work queues are picked by must spread the workgroups over the XCDs."""
 import ctypes as C
 import os
 
@@ -31,3 +32,18 @@ def test_sqrt_correctly_rounded_and_rcp():
     assert bad.size == 0, f"{bad.size} sqrt results not correctly rounded, e.g. x={x[bad[0]]!r}"
     ulp = np.abs(r - 1.0 / x) / np.spacing(1.0 / x)
     assert ulp.max() <= 1.0
+
+
+def test_xcc_ids_spread_over_the_xcds():
+    """The eval kernel starts each wave on the work queue of its XCD (hardware register XCC_ID);
+    on an MI355X the workgroups of a launch see all eight XCDs."""
+    lib = C.CDLL(PROBE)
+    lib.probe_xcc.argtypes = [C.POINTER(C.c_int), C.c_int]
+    lib.probe_xcc.restype = C.c_int
+    n = 4096
+    ids = np.zeros(n, np.int32)
+    assert lib.probe_xcc(ids.ctypes.data_as(C.POINTER(C.c_int)), n) == 0
+    counts = np.bincount(ids, minlength=16)
+    print("XCC_ID counts:", counts.tolist())
+    assert ids.min() >= 0 and ids.max() < 8
+    assert (counts[:8] > 0).sum() == 8

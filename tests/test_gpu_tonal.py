@@ -145,3 +145,13 @@ def test_errors_are_returned(ctx):
     with pytest.raises(acum.AcumError) as e:
         ctx.tonal(c)
     assert e.value.code == acum.ACUM_ERR_ARG
+    # a non-finite load or position anywhere in the element data is reported, in both precisions
+    for field, idx in (("PR", 17), ("X", 4 * 5 + 1)):
+        for prec in (acum.FP64, acum.MIXED):
+            run, c = small_case(1, prec)
+            getattr(c, field)[idx] = float("nan")
+            with pytest.raises(acum.AcumError) as e:
+                ctx.tonal(c)
+            assert e.value.code == acum.ACUM_ERR_NUMERIC
+    run, c = small_case(1, acum.FP64)
+    ctx.tonal(c)  # the context keeps working after those errors
