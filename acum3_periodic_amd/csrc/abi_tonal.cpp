@@ -142,20 +142,25 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   // The CPU oracle builds the same constants with the same operations.
   TimeConst k{};
   k.a0 = c->a0;
-  k.inv_a0 = 1.0 / c->a0;
   k.r2b = (c->oM / c->a0) / c->dPsi;
   k.KB = c->oM / c->dPsi;
   k.SB = c->dTau * k.KB;
   k.nT = (int32_t)c->nTime;
-  k.vx = c->vInf[0];
-  k.vy = c->vInf[1];
-  k.vz = c->vInf[2];
-  double vv = k.vx * k.vx;
-  vv = vv + k.vy * k.vy;
-  vv = vv + k.vz * k.vz;
-  k.na = k.a0 * k.a0 - vv;
-  k.inv_na = 1.0 / k.na;
-  k.still = (k.vx == 0.0 && k.vy == 0.0 && k.vz == 0.0) ? 1 : 0;
+  const double vx = c->vInf[0], vy = c->vInf[1], vz = c->vInf[2];
+  double vv = vx * vx;
+  vv = vv + vy * vy;
+  vv = vv + vz * vz;
+  const double na = k.a0 * k.a0 - vv;
+  const double kn = k.KB / na;
+  k.wx = vx * kn;
+  k.wy = vy * kn;
+  k.wz = vz * kn;
+  k.C2 = k.KB * kn;
+  k.aB = k.a0 / k.KB;
+  k.ux = vx / k.KB;
+  k.uy = vy / k.KB;
+  k.uz = vz / k.KB;
+  k.still = (vx == 0.0 && vy == 0.0 && vz == 0.0) ? 1 : 0;
   return k;
 }
 
@@ -617,11 +622,11 @@ int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
 
 acum_tune default_tune(int prec) {
   acum_tune t{};
-  // measured on the four-blade rotor (4 x 20,000 elements, 1024 steps, 256 observers): the
-  // eval kernel's occupancy is bound by its LDS window, and a 24-bin window with 3-bin tiles ran
-  // fastest in both precisions (0.10 s mixed, 0.20 s FP64 per pass, against 0.21 s and 0.27 s
-  // with 64 and 48 bins)
-  t.window = 24;
+  // measured on the four-blade rotor (4 x 20,000 elements, 1024 steps, 1024 observers): the
+  // eval kernel's occupancy is bound by its LDS window (16 one-wave workgroups per CU at 20
+  // bins), and a 20-bin window with 3-bin tiles ran fastest in both precisions (0.26 s mixed,
+  // 0.59 s FP64 per pass, against 0.29 s and 0.66 s with 24 bins and 0.49 s and 0.80 s with 16)
+  t.window = 20;
   t.max_spread = 3;
   t.max_tile = 256;
   t.groups = 0;
@@ -824,7 +829,7 @@ int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* b
   const bool mixed = precision == ACUM_MIXED;
   std::vector<acum_tune> cand;
   // the window sets the occupancy and the spread the tile size; the tile length matters little
-  for (int W : {16, 24, 32, 48}) {
+  for (int W : {16, 20, 24, 32}) {
     if (!eval_window_supported(mixed, W)) continue;
     for (int sp : {2, 3, 4})
       for (int mt : {256}) {

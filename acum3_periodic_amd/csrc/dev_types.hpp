@@ -12,7 +12,7 @@ namespace acum {
 // from the parsed floats and stored in the path's precision R:
 //   v   = V / a0,  vd = dV/dt / a0                        (Mr, dMr of core.cu:93-95)
 //   l   = P n,     ld = dP/dt n + P dn/dt                  (L, dL/dt of core.cu:130-131)
-//   aMM = a0 M^2 = |V|^2 / a0,  aLM = a0 L_M = P (V.n)     (core.cu:94,134)
+//   aM1 = a0 (M^2 - 1) = |V|^2 / a0 - a0,  aLM = a0 L_M = P (V.n)   (core.cu:94,134)
 //   cT  = rhoRef (V.n) dS a0 D1,  cL = dS D1,  D1 = 1/(4 pi a0)   (core.cu:53,123-124,138-141)
 struct PosRec {
   double x, y, z, jt;
@@ -20,26 +20,28 @@ struct PosRec {
 template <typename R>
 struct AttrT {
   R v[3], vd[3], l[3], ld[3];
-  R aMM, aLM, cT, cL;
+  R aM1, aLM, cT, cL;
 };  // 64 bytes (float), 128 bytes (double)
 
 // Canonical FP64 observer time, in units of the observer-time bin dPsi; the CPU oracle forms
 // the same values with the same operations (no contraction other than the explicit fma).
 //   d = obs - src;  c = fma(dz,dz, fma(dy,dy, dx*dx))                 |obs - src|^2
 //   still air:  s = sqrt(c);  tau = fma(s, r2b, jt)                  r = s
-//   moving:     b = fma(dz,vz, fma(dy,vy, dx*vx));  s = sqrt(fma(b, b, c*na));
-//               t = (s - b) * inv_na;  tau = fma(t, KB, jt)          r = a0 t
-// with na = a0^2 - |vinf|^2 > 0, r2b = (oM / a0) / dPsi, KB = oM / dPsi, jt = j (dTau KB).
-// t is the positive root (-b' - sqrt(b'^2 - 4 a' c)) / (2 a') of a' t^2 + b' t + c = 0,
-// a' = |vinf|^2 - a0^2 < 0, b' = -2 d.vinf (acous_dist, core.cu:261-279), rewritten without
-// the factors of 2 and the sign flips; tau is
-// (j dTau + r / a0) oM / dPsi (core.cu:156).  The bins of an eval are the integers b in
+//   moving:     b = fma(dz,wz, fma(dy,wy, dx*wx));  T = sqrt(fma(b, b, c*C2)) - b;
+//               tau = T + jt                                          r = aB T
+// with r2b = (oM / a0) / dPsi, KB = oM / dPsi, jt = j (dTau KB), na = a0^2 - |vinf|^2 > 0,
+// w = vinf (KB / na), C2 = KB (KB / na), aB = a0 / KB.  T = KB t is the delay in bins, t the
+// positive root (-b' - sqrt(b'^2 - 4 a' c)) / (2 a') of a' t^2 + b' t + c = 0, a' = |vinf|^2 -
+// a0^2 < 0, b' = -2 d.vinf (acous_dist, core.cu:261-279): t = (s - d.vinf) / na with
+// s = sqrt((d.vinf)^2 + c na), so KB t = sqrt((KB d.vinf / na)^2 + c KB^2 / na) - KB d.vinf / na.
+// tau is (j dTau + r / a0) oM / dPsi (core.cu:156).  The bins of an eval are the integers b in
 // (tau_prev, tau] (rising side of the tent) and in (tau, tau_next) (falling side); bin b is
 // observer time b dPsi, stored at b mod nT.
 struct TimeConst {
-  double a0, inv_a0, r2b, KB, SB;
-  double vx, vy, vz, na, inv_na;
-  int32_t nT, still;  // still: vinf == 0
+  double a0, r2b, KB, SB;
+  double wx, wy, wz, C2;  // moving medium: canonical observer time
+  double aB, ux, uy, uz;  // moving medium: r = aB T, r_vec = d - u T with u = vinf / KB
+  int32_t nT, still;      // still: vinf == 0
 };
 
 struct TonalCaseDev {

@@ -66,7 +66,7 @@ __device__ __forceinline__ T load_words(kptr<T> p) {
 // ---------------------------------------------------------------------------------------------
 // Canonical FP64 observer time in bins (acous_dist core.cu:261-279, ttemp core.cu:156); the CPU
 // oracle evaluates the same operations in the same order.  `delay` returns r (still air, metres)
-// or t = r / a0 (moving medium, seconds).
+// or T = KB r / a0 (moving medium, the delay in bins).
 template <bool MOVING>
 __device__ __forceinline__ double obs_time(const TimeConst& k, double jt, double dx, double dy,
                                            double dz, double* delay) {
@@ -76,11 +76,10 @@ __device__ __forceinline__ double obs_time(const TimeConst& k, double jt, double
     *delay = s;
     return __builtin_fma(s, k.r2b, jt);
   } else {
-    const double b = __builtin_fma(dz, k.vz, __builtin_fma(dy, k.vy, dx * k.vx));
-    const double s = sqrt_pos(__builtin_fma(b, b, c * k.na));
-    const double t = (s - b) * k.inv_na;
-    *delay = t;
-    return __builtin_fma(t, k.KB, jt);
+    const double b = __builtin_fma(dz, k.wz, __builtin_fma(dy, k.wy, dx * k.wx));
+    const double T = sqrt_pos(__builtin_fma(b, b, c * k.C2)) - b;
+    *delay = T;
+    return T + jt;
   }
 }
 
@@ -90,8 +89,11 @@ __device__ __forceinline__ float fmaT(float a, float b, float c) { return __buil
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcpT(double x) { return rcp_nr(x); }
-__device__ __forceinline__ float maxT(float a, float b) { return __builtin_fmaxf(a, b); }
-__device__ __forceinline__ double maxT(double a, double b) { return __builtin_fmax(a, b); }
+// clamp into [0, 1]: folds into the clamp bit of the instruction that produces x
+__device__ __forceinline__ float clamp01(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 1.0f); }
+__device__ __forceinline__ double clamp01(double x) { return __builtin_fmin(__builtin_fmax(x, 0.0), 1.0); }
+__device__ __forceinline__ float fractT(double x, float) { return (float)__builtin_amdgcn_fract(x); }
+__device__ __forceinline__ double fractT(double x, double) { return __builtin_amdgcn_fract(x); }
 
 template <bool MIXED>
 struct Prec {
@@ -116,17 +118,19 @@ struct Prec<true> {
 // products (AttrT) dotted into r_vec -- Mr = dV/r, dMr = dVd/r, Lr = dL/r, dLr = dLd/r for
 // dV = v.r_vec, dVd = vd.r_vec, dL = l.r_vec, dLd = ld.r_vec -- and g = 1/(r (1 - Mr)) =
 // 1/(r - dV), every 1/r cancels:
-//   pThick = cT g^3 (r (dVd - a0 M^2) + a0 dV)
-//   pLoad  = cL g^2 (dL g (dVd - a0 M^2 + a0) + dLd - a0 L_M)
-// (using 1 + g dV = r g), one reciprocal per eval.  d is obs - src; `del` is r in still air and
-// t = r / a0 in a moving medium, where r_vec = d - vinf t (core.cu:86).  `bad` marks r <= 0 or
-// 1 - Mr <= 0 (and NaN).
+//   pThick = cT g^3 (r (dVd - a0 M^2) + a0 dV) = cT g^2 (g r w - a0)
+//   pLoad  = cL g^2 (dL g (dVd - a0 M^2 + a0) + dLd - a0 L_M) = cL g^2 (dL g w + dLd - a0 L_M)
+// with w = dVd - a0 (M^2 - 1) (using 1 + g dV = r g and g (r - dV) = 1), one reciprocal per
+// eval.  d is obs - src; `del` is r in still air and the delay T in bins in a moving medium,
+// where r = aB T and r_vec = d - u T (core.cu:86; aB = a0 / KB, u = vinf / KB).  Returns r (1 - Mr):
+// positive for a valid eval, zero for a source on the observer (r = 0 gives r_vec = 0 and dV = 0)
+// and negative for Mr > 1.
 template <typename R, bool MOVING>
-__device__ __forceinline__ void integrand(const AttrT<R>& q, R dx, R dy, R dz, R del, R a0,
-                                          R vx, R vy, R vz, R* pth, R* plo, bool* bad) {
+__device__ __forceinline__ R integrand(const AttrT<R>& q, R dx, R dy, R dz, R del, R a0, R aB,
+                                       R vx, R vy, R vz, R* pth, R* plo) {
   R r, rx, ry, rz;
   if (MOVING) {
-    r = a0 * del;
+    r = aB * del;
     rx = fmaT(-vx, del, dx);
     ry = fmaT(-vy, del, dy);
     rz = fmaT(-vz, del, dz);
@@ -143,11 +147,17 @@ __device__ __forceinline__ void integrand(const AttrT<R>& q, R dx, R dy, R dz, R
   const R rm = r - dV;                // r (1 - Mr)
   const R g = rcpT(rm);
   const R g2 = g * g;
-  const R u = dVd - q.aMM;
-  *pth = (q.cT * g2) * (g * fmaT(r, u, a0 * dV));
-  *plo = (q.cL * g2) * fmaT(dL * g, u + a0, dLd - q.aLM);
-  *bad = !(r > (R)0) || !(rm > (R)0);
+  const R w = dVd - q.aM1;
+  *pth = (q.cT * g2) * fmaT(g * r, w, -a0);
+  *plo = (q.cL * g2) * fmaT(dL * g, w, dLd - q.aLM);
+  return rm;
 }
+
+// The high word of a finite value read as a signed integer is positive exactly when the value is
+// positive and at least the smallest normal number: a running integer minimum of these words over
+// r (1 - Mr) and the observer-time steps flags every invalid eval with one instruction.
+__device__ __forceinline__ int sign_word(float x) { return __float_as_int(x); }
+__device__ __forceinline__ int sign_word(double x) { return (int)(__double_as_longlong(x) >> 32); }
 
 // ---------------------------------------------------------------------------------------------
 // Eval + binning.  One wave per workgroup, persistent over a queue of items.
@@ -179,7 +189,12 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
   __shared__ R2 win[W * 64];
   const int lane = threadIdx.x;
   for (int s = 0; s < W; s++) win[s * 64 + lane] = R2{(R)0, (R)0};
-  uint64_t badmask = 0, winmask = 0, sweepmask = 0;  // lanes that raised each error
+  // Errors: `valid` is the minimum sign word of r (1 - Mr) and of every step tau_next - tau, so
+  // it stays positive while every eval is valid and observer time increases.  Increasing time
+  // makes the footprints of an element pass monotonic, so the window bound is checked on the
+  // pass's first and last footprints (each footprint is also clamped into the window).
+  int valid = 0x7fffffff;
+  uint64_t winmask = 0, sweepmask = 0;  // lanes that raised each error
   // start on this XCD's queue (hardware register XCC_ID, bits 3:0), then drain the others
   int q = (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & (kQueues - 1));
   int drained = 0;
@@ -195,7 +210,10 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
     }
     const Item item = items[idx];
     const TonalCaseDev* Cp = cases + item.icase;
-    const TimeConst k = Cp->k;
+    TimeConst k = Cp->k;
+    // keep the observer-time factors in vector registers: an FMA may read one scalar operand,
+    // and the other is the record's jt
+    __asm__("" : "+v"(k.r2b));
     const int nT = k.nT, nObs = Cp->nObs;
     const int o = item.obs0 + lane;
     const bool active = o < nObs;
@@ -204,7 +222,7 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
     R* slab = (R*)Cp->slab + ((int64_t)item.group * nObs + oc) * (int64_t)nT * 2;
     const PosPtr pos = (PosPtr)(uintptr_t)Cp->pos;
     const AttrPtr attr = (AttrPtr)(uintptr_t)Cp->attr;
-    const R a0 = (R)k.a0, vx = (R)k.vx, vy = (R)k.vy, vz = (R)k.vz;
+    const R a0 = (R)k.a0, aB = (R)k.aB, ux = (R)k.ux, uy = (R)k.uy, uz = (R)k.uz;
     const double dnT = (double)nT;
     for (int ti = item.tile0; ti < item.tile0 + item.ntile; ti++) {
       const Tile tl = tiles[ti];
@@ -223,39 +241,52 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
           const PosPtr pe = pos + e * nT;
           const AttrPtr ae = attr + e * nT;
-          // state at the start of the pass: tau(k0-1), tau(k0)
-          double t_prev, t_cur, dl;
+          // Per-step state, kept twice and used in turn (cur is the step being evaluated, prv the
+          // one before it), as are the record buffers: the loop runs two steps per iteration with
+          // the roles swapped, so nothing is copied from one step to the next.
+          struct Step {
+            double t;  // observer time tau in bins
+            int ic;    // floor(tau)
+            R id;      // 1 / (tau - tau of the step before)
+          };
+          Step s0, s1;
+          R dxc, dyc, dzc, delc;  // obs - src and the delay of the step being evaluated
           {
+            double dl;
             const PosRec p = load_words<PosRec>(pe + km0);
-            t_prev = obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &dl);
-            if (k0 == 0) t_prev = t_prev - dnT;
-          }
-          R dxc, dyc, dzc, delc;
-          {
-            const PosRec p = load_words<PosRec>(pe + k0);
-            const double dx = ox - p.x, dy = oy - p.y, dz = oz - p.z;
-            t_cur = obs_time<MOVING>(k, p.jt, dx, dy, dz, &dl);
+            double tp = obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &dl);
+            if (k0 == 0) tp = tp - dnT;  // the step before step 0 is step nT-1 of the previous period
+            const PosRec q = load_words<PosRec>(pe + k0);
+            const double dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
+            const double tc = obs_time<MOVING>(k, q.jt, dx, dy, dz, &dl);
             dxc = (R)dx;
             dyc = (R)dy;
             dzc = (R)dz;
             delc = (R)dl;
+            const R dp = (R)(tc - tp);
+            s1 = Step{tp, (int)floor(tp), (R)0};
+            // tau >= 0 from here on (only the step before step 0 carries the -nT wrap), so
+            // truncation is floor
+            s0 = Step{tc, (int)tc, rcpT(dp)};
+            valid = min(valid, sign_word(dp));
           }
-          double fl_c = floor(t_cur);
-          R idp = rcpT((R)(t_cur - t_prev));  // 1 / (tau - tau_prev), carried from step to step
+          winmask |= __ballot(s0.ic - 1 - base < 0);  // the pass's first footprint
+          // lanes whose current step has a rising side longer than two bins
+          uint64_t longp = __ballot(s1.ic < s0.ic - 1 && s0.t - s1.t > 1.999);
           // record pipeline: positions one step ahead of the solve, attributes one step ahead
           // of the integrand, each with one load in flight
-          PosRec pn = load_words<PosRec>(pe + kn0);
-          Attr ac = load_words<Attr>(ae + k0);
-          auto step = [&](int kk, bool wrap) __attribute__((always_inline)) {
+          PosRec p0 = load_words<PosRec>(pe + kn0), p1;
+          Attr a0r = load_words<Attr>(ae + k0), a1r;
+          auto step = [&](int kk, bool wrap, Step& cur, Step& prv, const PosRec& pn, PosRec& pnn,
+                          const Attr& ac, Attr& an) __attribute__((always_inline)) {
             const int kn = (kk + 1 < nT) ? kk + 1 : 0;
             const int knn = (kn + 1 < nT) ? kn + 1 : 0;
-            const PosRec pnn = load_words<PosRec>(pe + knn);
-            const Attr an = load_words<Attr>(ae + kn);
+            pnn = load_words<PosRec>(pe + knn);
+            an = load_words<Attr>(ae + kn);
             // the footprint is known before the solve: start its window reads now
-            const int ic = (int)fl_c;
-            const int sb = ic - 1 - base;
-            const bool inrange = (unsigned)sb <= (unsigned)(W - 4);
-            R2* wp = &win[(inrange ? sb : 0) * 64 + lane];
+            const int ic = cur.ic;
+            const int sb = min(max(ic - 1 - base, 0), W - 4);
+            R2* wp = &win[sb * 64 + lane];
             R2 v0 = wp[0], v1 = wp[64], v2 = wp[128], v3 = wp[192];
             // keep the record loads and window reads at the top of the step: their latency then
             // overlaps the solve and the integrand instead of the window update
@@ -267,20 +298,17 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
             if (wrap) t_next = t_next + dnT;
             // integrand of this step
             R pth, plo;
-            bool bad;
-            integrand<R, MOVING>(ac, dxc, dyc, dzc, delc, a0, vx, vy, vz, &pth, &plo, &bad);
-            badmask |= __ballot(bad);
-            winmask |= __ballot(!inrange);
+            const R rm = integrand<R, MOVING>(ac, dxc, dyc, dzc, delc, a0, aB, ux, uy, uz, &pth, &plo);
             // tent weights of the footprint
-            const R fr = (R)(t_cur - fl_c);
-            const R dp = (R)(t_cur - t_prev), dn = (R)(t_next - t_cur);
-            const R idn = rcpT(dn);
-            const R wr0 = fmaT(-fr, idp, (R)1);        // bin ic
-            const R wf0 = fmaT(fr - (R)1, idn, (R)1);  // bin ic+1
-            const R w0 = maxT(wr0 - idp, (R)0);        // bin ic-1
-            const R w1 = maxT(wr0, (R)0);
-            const R w2 = maxT(wf0, (R)0);
-            const R w3 = maxT(wf0 - idn, (R)0);        // bin ic+2
+            const R fr = fractT(cur.t, R());  // tau - ic, exact
+            const R dn = (R)(t_next - cur.t);
+            valid = min(valid, min(sign_word(rm), sign_word(dn)));
+            const R idp = cur.id, idn = rcpT(dn);
+            const R frm = fr - (R)1;
+            const R w0 = clamp01(fmaT(-fr, idp, (R)1 - idp));  // bin ic-1: 1 - (fr + 1) / dp
+            const R w1 = clamp01(fmaT(-fr, idp, (R)1));        // bin ic:   1 - fr / dp
+            const R w2 = clamp01(fmaT(frm, idn, (R)1));        // bin ic+1: 1 - (1 - fr) / dn
+            const R w3 = clamp01(fmaT(frm, idn, (R)1 - idn));  // bin ic+2: 1 - (2 - fr) / dn
             v0.x = fmaT(pth, w0, v0.x);
             v0.y = fmaT(plo, w0, v0.y);
             v1.x = fmaT(pth, w1, v1.x);
@@ -293,11 +321,17 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
             wp[64] = v1;
             wp[128] = v2;
             wp[192] = v3;
-            // a side longer than two bins can hold a third bin
-            const bool tail = dp > (R)1.999 || dn > (R)1.999;
-            if (wave_any(tail)) {
-              if (tail) {
-                const int nd = ic - (int)floor(t_prev);
+            // lanes with a side longer than two bins; a step's rising side is the previous step's
+            // falling side, so one compare per step carries both
+            const uint64_t longn = __ballot(dn > (R)1.999);
+            const uint64_t tailm = longn | longp;
+            longp = longn;
+            if (tailm) {
+              // keeps this wave-uniform test a branch of its own (the lane test below then costs
+              // nothing on the common path)
+              __asm__ volatile("");
+              if ((tailm >> lane) & 1) {
+                const int nd = ic - prv.ic;
                 const int nu = (int)ceil(t_next) - 1 - ic;
                 if (nd > W || nu > W) sweepmask |= 1;
                 const int nd_ = min(nd, W), nu_ = min(nu, W);
@@ -316,21 +350,36 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
                 for (int j = 2; j < nu_; j++) put(ic + 1 + j, fmaT(fr - (R)(1 + j), idn, (R)1));
               }
             }
-            t_prev = t_cur;
-            t_cur = t_next;
-            idp = idn;
-            fl_c = floor(t_next);
+            // the step before becomes the next step
+            prv.t = t_next;
+            prv.ic = (int)t_next;
+            prv.id = idn;
             dxc = (R)dxn;
             dyc = (R)dyn;
             dzc = (R)dzn;
             delc = (R)dln;
-            pn = pnn;
-            ac = an;
           };
-          // the last step of the period wraps tau_next by +nT; it is peeled off the loop
+          // the last step of the period wraps tau_next by +nT; it and an odd step run singly,
+          // each followed by a swap of the buffers (once or twice per pass)
           const int kstop = (kend == nT) ? nT - 1 : kend;
-          for (int kk = k0; kk < kstop; kk++) step(kk, false);
-          if (kend == nT) step(nT - 1, true);
+          int kk = k0;
+          for (; kk + 1 < kstop; kk += 2) {
+            step(kk, false, s0, s1, p0, p1, a0r, a1r);
+            step(kk + 1, false, s1, s0, p1, p0, a1r, a0r);
+          }
+          for (; kk < kend; kk++) {
+            step(kk, kk == nT - 1, s0, s1, p0, p1, a0r, a1r);
+            const Step ts = s0;
+            s0 = s1;
+            s1 = ts;
+            const PosRec tp = p0;
+            p0 = p1;
+            p1 = tp;
+            const Attr ta = a0r;
+            a0r = a1r;
+            a1r = ta;
+          }
+          winmask |= __ballot(s1.ic - 1 - base > W - 4);  // the pass's last footprint
         }
         // add the window into this item's slab row and clear it (untouched slots hold zeros)
         int b = base % nT;
@@ -349,8 +398,8 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
       }
     }
   }
-  const uint32_t flags = (badmask ? kFlagNumeric : 0u) | (winmask ? kFlagWindow : 0u) |
-                         (sweepmask ? kFlagSweep : 0u);
+  const uint32_t flags = (wave_any(valid <= 0) ? kFlagNumeric : 0u) |
+                         (winmask ? kFlagWindow : 0u) | (sweepmask ? kFlagSweep : 0u);
   if (flags && lane == 0) atomicOr(flags_out, flags);
 }
 
@@ -481,7 +530,7 @@ __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __re
       Vn += Vi * ni;
       VV += Vi * Vi;
     }
-    o.aMM = (R)(VV * inv_a0);
+    o.aM1 = (R)(VV * inv_a0 - a0);
     o.aLM = (R)(P * Vn);
     o.cT = (R)(rhoRef * Vn * ds * a0 * D1);
     o.cL = (R)(ds * D1);
@@ -552,7 +601,8 @@ __global__ void tonal_bins_kernel(const PosRec* __restrict__ pos, int64_t nE, Ti
 // ---------------------------------------------------------------------------------------------
 // Host-side launchers (called from the C-ABI layer).
 #define ACUM_EVAL_VARIANTS(X) \
-  X(true, 16) X(true, 24) X(true, 32) X(true, 48) X(true, 64) X(false, 16) X(false, 24) X(false, 32)
+  X(true, 16) X(true, 20) X(true, 24) X(true, 32) X(true, 48) X(true, 64) X(false, 16) X(false, 20) \
+  X(false, 24) X(false, 32)
 
 hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
                              const TonalCaseDev* cases, const Tile* tiles, const Item* items,

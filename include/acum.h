@@ -111,7 +111,7 @@ typedef struct acum_timing {
 
 /* Tile shapes of the tonal kernel (chosen by acum_autotune or set by the caller). */
 typedef struct acum_tune {
-  int32_t window;      /* observer-time bins per lane in LDS: 16, 24, 32 (+ 48, 64 mixed)  */
+  int32_t window;      /* observer-time bins per lane in LDS: 16, 20, 24, 32 (+ 48, 64 mixed) */
   int32_t max_spread;  /* largest spread (bins) of a tile's elements around its first one     */
   int32_t max_tile;    /* most elements per tile                                              */
   int32_t groups;      /* partial slabs per observer row (0 = automatic)                      */

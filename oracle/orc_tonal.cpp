@@ -39,8 +39,9 @@
 namespace {
 
 struct TimeConst {
-  double a0, inv_a0, r2b, KB, SB;
-  double vx, vy, vz, na, inv_na;
+  double a0, r2b, KB, SB;
+  double vx, vy, vz;
+  double wx, wy, wz, C2, aB;  // moving medium: w = vinf KB / na, C2 = KB^2 / na, aB = a0 / KB
   int64_t nT;
   bool still;  // vinf == 0
 };
@@ -48,7 +49,6 @@ struct TimeConst {
 TimeConst make_time_const(const acum_tonal_case* c) {
   TimeConst k;
   k.a0 = c->a0;
-  k.inv_a0 = 1.0 / c->a0;
   k.r2b = (c->oM / c->a0) / c->dPsi;
   k.KB = c->oM / c->dPsi;
   k.SB = c->dTau * k.KB;
@@ -59,14 +59,21 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   double vv = k.vx * k.vx;
   vv = vv + k.vy * k.vy;
   vv = vv + k.vz * k.vz;
-  k.na = k.a0 * k.a0 - vv;
-  k.inv_na = 1.0 / k.na;
+  const double na = k.a0 * k.a0 - vv;
+  const double kn = k.KB / na;
+  k.wx = k.vx * kn;
+  k.wy = k.vy * kn;
+  k.wz = k.vz * kn;
+  k.C2 = k.KB * kn;
+  k.aB = k.a0 / k.KB;
   k.still = (k.vx == 0.0 && k.vy == 0.0 && k.vz == 0.0);
   return k;
 }
 
 // Observer time of step j in bins and the retarded distance r (acous_dist, core.cu:261-279;
-// ttemp, core.cu:156).
+// ttemp, core.cu:156).  Moving medium: T = KB t, the delay in bins, is the positive root of
+// |d - vinf T / KB| = a0 T / KB, i.e. T = sqrt(b^2 + c KB^2 / na) - b with b = KB (d.vinf) / na
+// and na = a0^2 - |vinf|^2; tau = T + j SB and r = a0 T / KB.
 inline double obs_tau_r(const TimeConst& k, int64_t j, double ox, double oy, double oz,
                         const float* src, double* r) {
   const double dx = ox - (double)src[0];
@@ -79,11 +86,10 @@ inline double obs_tau_r(const TimeConst& k, int64_t j, double ox, double oy, dou
     *r = s;
     return std::fma(s, k.r2b, jt);
   }
-  const double b = std::fma(dz, k.vz, std::fma(dy, k.vy, dx * k.vx));
-  const double s = std::sqrt(std::fma(b, b, c * k.na));
-  const double t = (s - b) * k.inv_na;
-  *r = t * k.a0;
-  return std::fma(t, k.KB, jt);
+  const double b = std::fma(dz, k.wz, std::fma(dy, k.wy, dx * k.wx));
+  const double T = std::sqrt(std::fma(b, b, c * k.C2)) - b;
+  *r = T * k.aB;
+  return T + jt;
 }
 
 struct Eval {

@@ -107,8 +107,9 @@ def test_batch_equals_single(ctx):
 
 
 @pytest.mark.parametrize("prec,window,spread,tile", [
-    (acum.MIXED, 16, 2, 32), (acum.MIXED, 48, 6, 64), (acum.MIXED, 64, 8, 256),
-    (acum.FP64, 16, 1, 16), (acum.FP64, 32, 6, 128)])
+    (acum.MIXED, 16, 2, 32), (acum.MIXED, 24, 4, 256), (acum.MIXED, 48, 6, 64),
+    (acum.MIXED, 64, 8, 256), (acum.FP64, 16, 1, 16), (acum.FP64, 24, 3, 256),
+    (acum.FP64, 32, 6, 128)])
 def test_tile_shapes_agree(ctx, prec, window, spread, tile):
     """Window size, tile spread and tile length change only the partial-sum grouping."""
     run, c = small_case(2, prec)
