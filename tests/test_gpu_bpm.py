@@ -63,3 +63,34 @@ def test_bpm_deterministic(ctx):
     a = ctx.bpm(c)
     b = ctx.bpm(c)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("t_del", [1, 2])
+def test_bpm_clockwise_surfaces(ctx, t_del):
+    """Surfaces turning clockwise (ccw = 0: mirrored section y, negated azimuth and the reference's
+    normal-vector scaling, bb_core.cu:79-164,505-512) next to counter-clockwise ones."""
+    run = acum.Run.synth(acum.synth_params(3, nSect=10, nTime=20, nObsAz=4, nObsEl=3, trip=0, t_del=t_del))
+    c = run.bpm_case()
+    ccw = np.ctypeslib.as_array(c.ccw, shape=(c.nSurf_BB,))
+    ccw[1::2] = 0
+    gspl, goa = ctx.bpm(c)
+    ospl, ooa = O.bpm_f64(c)
+    assert np.abs(gspl.astype(np.float64) - ospl).max() <= SPL_DB
+    assert np.abs(goa - ooa).max() <= SPL_DB
+
+
+def test_bpm_table_errors_are_returned(ctx):
+    """A table without a negative angle of attack cannot give cl(0): the call fails cleanly and
+    the context keeps working."""
+    run = acum.Run.synth(acum.synth_params(3, nSect=6, nTime=12, nObsAz=3, nObsEl=2))
+    c = run.bpm_case()
+    nS = c.nSurf_BB
+    ext = np.ctypeslib.as_array(c.aPtr, shape=(nS,)) + np.ctypeslib.as_array(c.naoa_t, shape=(nS,))
+    aoa = np.ctypeslib.as_array(c.aoa_t, shape=(int(ext.max()),))
+    keep = aoa.copy()
+    aoa[:] = np.abs(aoa) + 1.0
+    with pytest.raises(acum.AcumError) as e:
+        ctx.bpm(c)
+    assert e.value.code == acum.ACUM_ERR_ARG
+    aoa[:] = keep
+    ctx.bpm(c)
