@@ -116,6 +116,10 @@ def _load():
         "acum_synth_batch_case": (C.c_int, [C.c_uint64, C.c_int32, C.POINTER(SynthParams)]),
         "acum_synth": (C.c_int, [C.POINTER(SynthParams), C.POINTER(P)]),
         "acum_oaspl": (C.c_int, [C.c_int64, C.c_int64, _f32p, C.POINTER(C.c_double)]),
+        "acum_tonal_spectra": (C.c_int, [P, C.c_int64, C.c_int64, _f32p, C.c_double, C.c_int32, _f32p, _f32p,
+                                         _f32p]),
+        "acum_tonal_fetch_spectra": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, _f32p, _f32p, _f32p]),
+        "acum_run_stage": (C.c_int, [P, P, _i32p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -134,7 +138,8 @@ EXPORTS = ("acum_device_count", "acum_ctx_create", "acum_ctx_destroy", "acum_las
            "acum_run_free", "acum_run_preprocess", "acum_preprocess_gpu", "acum_run_inputs", "acum_run_tonal_case",
            "acum_run_bpm_case", "acum_run_write_tonal", "acum_run_write_bb", "acum_run_save",
            "acum_run_surface_raw", "acum_synth_defaults", "acum_synth_batch_case", "acum_synth",
-           "acum_oaspl")
+           "acum_oaspl", "acum_tonal_spectra", "acum_tonal_fetch_spectra",
+           "acum_run_stage")
 
 
 class AcumError(RuntimeError):
@@ -316,6 +321,33 @@ class Context:
 
     def release(self, handle):
         self._check(lib.acum_tonal_release(self._h, handle))
+
+    def stage_run(self, run):
+        """Stages a run's tonal case; a run not yet preprocessed is preprocessed on the GPU in
+        place (its element data stay on the device).  Returns the handle."""
+        h = C.c_int32()
+        self._check(lib.acum_run_stage(self._h, run._h, C.byref(h)))
+        return h.value
+
+    def spectra(self, pA, dt, fc):
+        """Band levels [nObs, nBands] and overall levels [nObs] (dB re 20 uPa) of a [nTime, nObs]
+        pressure history with observer-time step dt (s), bands centred at fc (Hz)."""
+        pA = np.ascontiguousarray(pA, np.float32)
+        fc = np.ascontiguousarray(fc, np.float32)
+        nT, nO = pA.shape
+        spl = np.empty((nO, fc.size), np.float32)
+        oa = np.empty(nO, np.float32)
+        self._check(lib.acum_tonal_spectra(self._h, nT, nO, fptr(pA), float(dt), fc.size, fptr(fc), fptr(spl),
+                                           fptr(oa)))
+        return spl, oa
+
+    def fetch_spectra(self, handle, icase, nObs, fc):
+        """spectra() of case icase of a staged batch, from its device-resident pA."""
+        fc = np.ascontiguousarray(fc, np.float32)
+        spl = np.empty((nObs, fc.size), np.float32)
+        oa = np.empty(nObs, np.float32)
+        self._check(lib.acum_tonal_fetch_spectra(self._h, handle, icase, fc.size, fptr(fc), fptr(spl), fptr(oa)))
+        return spl, oa
 
     def bins(self, case, iobs):
         nE = sum(case.E[s] for s in range(case.nSurf))

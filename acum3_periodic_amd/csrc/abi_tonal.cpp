@@ -11,8 +11,11 @@
 #include <string>
 #include <vector>
 #include "abi_internal.hpp"
+#include "host/run.hpp"
+#include "prep_types.hpp"
 
 namespace acum {
+hipError_t launch_preprocess(hipStream_t s, const PrepArgs& a, int grid);
 hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
                              const TonalCaseDev* cases, const Tile* tiles, const Item* items,
                              const int* qoff, int* counter, uint32_t* flags);
@@ -230,6 +233,90 @@ int stage_case(acum_ctx* ctx, const acum_tonal_case* c, StagedCase& sc) {
   HIPCHK(ctx, hipMemcpyAsync(sc.d_obs, c->rObs, (size_t)c->nObs * 3 * sizeof(float),
                              hipMemcpyHostToDevice, ctx->stream));
   for (int i = 0; i < 3; i++) HIPCHK(ctx, hipMalloc(&sc.d_out[i], (size_t)nT * c->nObs * sizeof(float)));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return ACUM_OK;
+}
+
+// A run read from its files but not preprocessed: the raw element arrays are uploaded once,
+// preprocess_imper (preprocess.cu:228-413) runs on the device in place (the same kernel as
+// acum_preprocess_gpu, bit-identical to the host FP64 form), and the staged case keeps the
+// device arrays, so the preprocessed element data never return to the host.
+int stage_raw_run(acum_ctx* ctx, const acum_run* r, StagedCase& sc) {
+  acum_tonal_case c{};
+  c.nSurf = r->nSurf;
+  c.nTime = r->nTime;
+  c.nObs = r->nObs;
+  c.E = r->E.data();
+  c.X = r->X.data();
+  c.N = r->N.data();
+  c.V = r->V.data();
+  c.PR = r->PR.data();
+  c.XPtr = r->XPtr.data();
+  c.NPtr = r->NPtr.data();
+  c.VPtr = r->VPtr.data();
+  c.PRPtr = r->PRPtr.data();
+  c.rObs = r->rObs.data();
+  c.dPsi = r->in.dPsi;
+  c.dTau = r->in.dTau;
+  c.a0 = r->in.a0;
+  c.rhoRef = r->in.rhoRef;
+  c.oM = r->in.oM;
+  for (int i = 0; i < 3; i++) c.vInf[i] = r->in.vInf[i];
+  c.precision = ACUM_MIXED;
+  int rc = validate(ctx, &c);
+  if (rc) return rc;
+  const int64_t nT = c.nTime;
+  if (nT > kPrepMaxSteps)
+    return ctx_fail(ctx, ACUM_ERR_ARG, "GPU preprocessing holds at most 4096 steps per series");
+  int64_t nE = 0;
+  sc.surf0.resize(c.nSurf);
+  for (int64_t s = 0; s < c.nSurf; s++) {
+    if (c.XPtr[s] != nE * nT * 4 || c.NPtr[s] != nE * nT * 6 || c.VPtr[s] != nE * nT * 6 ||
+        c.PRPtr[s] != nE * nT * 2)
+      return ctx_fail(ctx, ACUM_ERR_ARG, "surface arrays are not contiguous");
+    sc.surf0[s] = nE;
+    nE += c.E[s];
+  }
+  sc.nT = nT;
+  sc.nObs = c.nObs;
+  sc.k = make_time_const(&c);
+  sc.rhoRef = c.rhoRef;
+  sc.Minf = std::sqrt(c.vInf[0] * c.vInf[0] + c.vInf[1] * c.vInf[1] + c.vInf[2] * c.vInf[2]) / c.a0;
+  sc.nE = nE;
+  const int64_t q = nE * nT;
+  const int width[4] = {4, 6, 6, 2};
+  const float* src[4] = {c.X, c.N, c.V, c.PR};
+  for (int a = 0; a < 4; a++) {
+    HIPCHK(ctx, hipMalloc(&sc.d_aos[a], (size_t)q * width[a] * sizeof(float)));
+    if (a != 2)  // V is written by the preprocessing
+      HIPCHK(ctx, hipMemcpyAsync(sc.d_aos[a], src[a], (size_t)q * width[a] * sizeof(float),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  }
+  PrepArgs pa{};
+  pa.X = sc.d_aos[0];
+  pa.N = sc.d_aos[1];
+  pa.V = sc.d_aos[2];
+  pa.PR = sc.d_aos[3];
+  pa.nE = nE;
+  pa.nT = (int)nT;
+  pa.periodic = r->in.periodic;
+  pa.lowpass = r->in.lowpass;
+  pa.C = r->in.CFDscale;
+  pa.pscale = 0.5 * r->in.rhoRef * (r->in.Mref * r->in.Mref) * (r->in.a0 * r->in.a0);
+  pa.dt = r->in.dTau;
+  acum_host::gauss_weights13(pa.g);
+  HIPCHK(ctx, launch_preprocess(ctx->stream, pa, (int)std::min<int64_t>(nE, 1 << 20)));
+  // host positions for the planner: the preprocessing scales them by CFDscale in FP64 and rounds
+  // to float, which the host repeats exactly
+  sc.pos.resize((size_t)q * 3);
+  const double C = r->in.CFDscale;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < q; i++)
+    for (int d = 0; d < 3; d++) sc.pos[i * 3 + d] = (float)((double)c.X[i * 4 + d] * C);
+  HIPCHK(ctx, hipMalloc(&sc.d_obs, (size_t)c.nObs * 3 * sizeof(float)));
+  HIPCHK(ctx, hipMemcpyAsync(sc.d_obs, c.rObs, (size_t)c.nObs * 3 * sizeof(float), hipMemcpyHostToDevice,
+                             ctx->stream));
+  for (int i = 0; i < 3; i++) HIPCHK(ctx, hipMalloc(&sc.d_out[i], (size_t)nT * c.nObs * sizeof(float)));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return ACUM_OK;
 }
@@ -711,6 +798,27 @@ int acum_tonal_stage(acum_ctx* ctx, const acum_tonal_case* cases, int32_t nCases
   return ACUM_OK;
 }
 
+int acum_run_stage(acum_ctx* ctx, const acum_run* run, int32_t* handle) {
+  if (!ctx || !run || !handle) return ctx_fail(ctx, ACUM_ERR_ARG, "bad argument");
+  if (run->preprocessed) {
+    acum_tonal_case c{};
+    if (acum_run_tonal_case(run, ACUM_MIXED, &c) != ACUM_OK)
+      return ctx_fail(ctx, ACUM_ERR_ARG, "run cannot describe a tonal case");
+    return acum_tonal_stage(ctx, &c, 1, handle);
+  }
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto st = std::make_unique<Stage>();
+  st->cases.resize(1);
+  int rc = stage_raw_run(ctx, run, st->cases[0]);
+  if (rc) {
+    free_stage(*st);
+    return rc;
+  }
+  *handle = ctx->next_handle++;
+  ctx->stages[*handle] = std::move(st);
+  return ACUM_OK;
+}
+
 int acum_tonal_run(acum_ctx* ctx, int32_t handle, int32_t precision, int32_t repeats, int32_t use_graph) {
   if (!ctx) return ACUM_ERR_ARG;
   Stage* st = get_stage(ctx, handle);
@@ -730,6 +838,17 @@ int acum_tonal_fetch(acum_ctx* ctx, int32_t handle, int32_t icase, float* pT, fl
   for (int i = 0; i < 3; i++)
     if (dst[i]) HIPCHK(ctx, hipMemcpy(dst[i], sc.d_out[i], n, hipMemcpyDeviceToHost));
   return ACUM_OK;
+}
+
+int acum_tonal_fetch_spectra(acum_ctx* ctx, int32_t handle, int32_t icase, int32_t nBands, const float* fc,
+                             float* spl, float* oaspl) {
+  if (!ctx) return ACUM_ERR_ARG;
+  Stage* st = get_stage(ctx, handle);
+  if (!st || icase < 0 || icase >= (int)st->cases.size())
+    return ctx_fail(ctx, ACUM_ERR_ARG, "unknown handle or case");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  StagedCase& sc = st->cases[icase];
+  return spectra_device(ctx, sc.d_out[2], sc.nT, sc.nObs, 1.0 / sc.k.KB, nBands, fc, spl, oaspl);
 }
 
 int acum_tonal_release(acum_ctx* ctx, int32_t handle) {

@@ -40,6 +40,9 @@ FLOP_PER_EVAL = 100.0          # algorithmic count per (element, step, observer)
 FLOP_FP32, FLOP_FP64 = 70.0, 30.0
 PEAK_FP32, PEAK_FP64 = 157.3e12, 78.6e12   # MI355X vector peaks, FLOP/s
 HBM_PEAK = 8.0e12                           # B/s
+# ISO 1/3-octave band centres, 10 Hz - 20 kHz (the bands of the tonal spectra leg)
+ISO_BANDS = (10, 12.5, 16, 20, 25, 31.5, 40, 50, 63, 80, 100, 125, 160, 200, 250, 315, 400, 500, 630, 800,
+             1000, 1250, 1600, 2000, 2500, 3150, 4000, 5000, 6300, 8000, 10000, 12500, 16000, 20000)
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "tonal_eval_c2_hbm_traffic.md")
 EVAL_KERNEL = "tonal_eval_kernel"
 CALIB_BYTES = 1 << 30
@@ -479,6 +482,12 @@ def main():
     items = tm.n_items
     tune = ctx.get_tuning(acum.MIXED)
 
+    # post-processing of the mixed history on the GPU: 1/3-octave band levels from the resident pA
+    note("tonal band spectra")
+    iso = np.array(ISO_BANDS, np.float32)
+    sp_spl, _ = ctx.fetch_spectra(h, 0, case.nObs, iso)
+    sp_ms = ctx.timing().eval_ms
+
     fp64 = None
     if not args.no_fp64:
         note("timing the FP64 path")
@@ -509,6 +518,10 @@ def main():
         acc["fp64_vs_oracle_rel_l2_pA"] = O.rel_l2(plain[2][:, :nob], oA)
         d = np.abs(acum.oaspl(mixed[2]) - acum.oaspl(plain[2]))
         acc["mixed_vs_fp64_all_obs_max_doaspl_db"] = float(d.max())
+
+    o_spl, _ = O.tonal_spectra(mixed[2][:, :nob], case.dPsi / case.oM, iso)
+    spectra = dict(kernels="spectra_bins + spectra_bands", ms=round(sp_ms, 3), bands=int(iso.size),
+                   max_dspl_db_vs_oracle=float(np.abs(sp_spl[:nob] - o_spl).max()))
 
     bpm = None
     if not args.no_bpm:
@@ -555,6 +568,7 @@ def main():
         "fp64_path": fp64,
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
         "accuracy": acc,
+        "spectra": spectra,
         "bpm": bpm,
         "preprocess": prep,
         "work_items": items,

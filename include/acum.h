@@ -173,6 +173,12 @@ int acum_run_preprocess(acum_run* run);
  * acum_run_preprocess (bit-identical results) and copied back.  nTime <= 4096.  Timing:
  * prep_ms = upload, eval_ms = kernel, reduce_ms = download.                                 */
 int acum_preprocess_gpu(acum_ctx* ctx, acum_run* run);
+/* Stages the tonal case of a run for acum_tonal_run / _fetch / _release (one case, handle out).
+ * A run that is not preprocessed is uploaded raw and preprocessed on the GPU in place
+ * (nTime <= 4096, as acum_preprocess_gpu, bit-identical to acum_run_preprocess), so the
+ * preprocessed element data never come back to the host; the run itself is left as it was.
+ * Replaces the host preprocess_imper call (main.cu:88) and the device copies (main.cu:156-234). */
+int acum_run_stage(acum_ctx* ctx, const acum_run* run, int32_t* handle);
 int acum_run_inputs(const acum_run* run, acum_inputs* in);
 int acum_run_tonal_case(const acum_run* run, int32_t precision, acum_tonal_case* c);
 int acum_run_bpm_case(const acum_run* run, acum_bpm_case* c);
@@ -206,6 +212,24 @@ int acum_synth(const acum_synth_params* p, acum_run** run);
 
 /* Tonal overall level per observer: 10 log10(mean_t (pA - mean pA)^2 / (2e-5)^2), dB.       */
 int acum_oaspl(int64_t nTime, int64_t nObs, const float* pA, double* oaspl);
+
+/* Tonal band spectra on the GPU.  No counterpart in the reference, which writes the pressure
+ * histories only (io.cu:137-203); this is the post-processing step that turns them into levels.
+ * Per observer, the one-sided mean-square of pA [nTime][nObs] (mean removed; DFT bins
+ * k = 1 .. nTime/2 at k / (nTime dt) Hz, each weighted 2 except the Nyquist bin) summed into the
+ * bands of centre frequencies fc[0 .. nBands) (strictly increasing, Hz).  Band b holds the bins
+ * with lo_b <= f < hi_b, where the edge between two bands is the geometric mean of their
+ * centres and the outer edges are fc[0] 2^(-1/6) and fc[nBands-1] 2^(1/6).
+ *   spl[o * nBands + b] = 10 log10(max(E_b, 1e-10 (2e-5)^2) / (2e-5)^2)   (-100 dB floor)
+ *   oaspl[o]            = 10 log10(variance / (2e-5)^2), as acum_oaspl
+ * dt is the observer-time step in seconds (dPsi / oM for the tonal outputs).  Over bands that
+ * cover every bin the band energies sum to the variance.                                      */
+int acum_tonal_spectra(acum_ctx* ctx, int64_t nTime, int64_t nObs, const float* pA, double dt,
+                       int32_t nBands, const float* fc, float* spl, float* oaspl);
+/* The same for case icase of a staged batch after acum_tonal_run, from its device-resident pA
+ * (dt = dPsi / oM of the case).                                                              */
+int acum_tonal_fetch_spectra(acum_ctx* ctx, int32_t handle, int32_t icase, int32_t nBands,
+                             const float* fc, float* spl, float* oaspl);
 
 #ifdef __cplusplus
 }

@@ -108,6 +108,53 @@ def bpm_f64(case, obs0=0, nobs=0, nthreads=0):
     return spl.reshape(n, case.nf2, 7), oa
 
 
+# ---- tonal band spectra (numpy restatement of the definition in include/acum.h) -------------
+SIXTH_DOWN = 0.8908987181403393  # 2^(-1/6), the same literal as the library's
+SIXTH_UP = 1.122462048309373     # 2^(1/6)
+PREF2 = 2e-5 * 2e-5
+
+
+def band_bins(nT, dt, fc):
+    """[lo_b, hi_b) DFT-bin range of each band: bin k (k / (nT dt) Hz) lies in band b when
+    edge_b <= k df < edge_{b+1}; inner edges are geometric means of adjacent centres."""
+    fc = np.asarray(fc, np.float32).astype(np.float64)
+    nbins = nT // 2
+    df = 1.0 / (float(nT) * dt)
+    edges = [fc[0] * SIXTH_DOWN] + [float(np.sqrt(fc[b - 1] * fc[b])) for b in range(1, fc.size)] + \
+            [fc[-1] * SIXTH_UP]
+    k = np.arange(1, nbins + 1, dtype=np.float64) * df
+
+    def first(f):
+        idx = np.nonzero(k >= f)[0]
+        return int(idx[0]) + 1 if idx.size else nbins + 1
+    lo = np.array([first(e) for e in edges[:-1]])
+    hi = np.array([first(e) for e in edges[1:]])
+    return lo, hi
+
+
+def bin_energy(pA):
+    """One-sided mean-square share of DFT bins k = 1 .. nT/2 of each column (mean removed)."""
+    x = np.asarray(pA, np.float64)
+    nT = x.shape[0]
+    x = x - x.mean(axis=0)
+    X = np.fft.rfft(x, axis=0)[1:nT // 2 + 1]
+    w = np.full(X.shape[0], 2.0)
+    if nT % 2 == 0:
+        w[-1] = 1.0
+    return w[:, None] * np.abs(X) ** 2 / float(nT) ** 2
+
+
+def tonal_spectra(pA, dt, fc):
+    """(spl [nObs, nBands], oaspl [nObs]) in dB re 20 uPa, FP64."""
+    x = np.asarray(pA, np.float64)
+    E = bin_energy(x)
+    lo, hi = band_bins(x.shape[0], dt, fc)
+    eb = np.stack([E[l - 1:h - 1].sum(axis=0) for l, h in zip(lo, hi)], axis=1)
+    spl = 10.0 * np.log10(np.maximum(eb, 1e-10 * PREF2) / PREF2)
+    var = ((x - x.mean(axis=0)) ** 2).mean(axis=0)
+    return spl, 10.0 * np.log10(var / PREF2)
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)

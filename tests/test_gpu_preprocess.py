@@ -61,3 +61,24 @@ def test_gpu_preprocess_limits(ctx):
     run = acum.Run.synth(acum.synth_params(1, nChord=2, nSpan=2, nTime=16)).preprocess()
     with pytest.raises(acum.AcumError):
         run.preprocess_gpu(ctx)
+
+
+@pytest.mark.parametrize("config,over", [(1, dict(nChord=4, nSpan=3, nTime=36, nObs=8)),
+                                         (2, dict(nChord=5, nSpan=4, nTime=64, nObsAz=6, nObsEl=3))])
+def test_stage_raw_run_equals_host_preprocessed(ctx, config, over):
+    """acum_run_stage on a run that was never preprocessed (preprocessing on the GPU, data kept on
+    the device) gives bit-identical histories to staging the host-preprocessed run."""
+    raw = acum.Run.synth(acum.synth_params(config, **over))
+    host = acum.Run.synth(acum.synth_params(config, **over)).preprocess()
+    c = host.tonal_case(acum.MIXED)
+    h_raw = ctx.stage_run(raw)
+    h_host = ctx.stage([c])
+    for prec in (acum.MIXED, acum.FP64):
+        ctx.run(h_raw, prec)
+        a = ctx.fetch(h_raw, 0, c.nTime, c.nObs)
+        ctx.run(h_host, prec)
+        b = ctx.fetch(h_host, 0, c.nTime, c.nObs)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+    ctx.release(h_raw)
+    ctx.release(h_host)
