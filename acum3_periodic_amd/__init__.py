@@ -120,6 +120,7 @@ def _load():
                                          _f32p]),
         "acum_tonal_fetch_spectra": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, _f32p, _f32p, _f32p]),
         "acum_run_stage": (C.c_int, [P, P, _i32p]),
+        "acum_run_save_cache": (C.c_int, [P, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -139,7 +140,7 @@ EXPORTS = ("acum_device_count", "acum_ctx_create", "acum_ctx_destroy", "acum_las
            "acum_run_bpm_case", "acum_run_write_tonal", "acum_run_write_bb", "acum_run_save",
            "acum_run_surface_raw", "acum_synth_defaults", "acum_synth_batch_case", "acum_synth",
            "acum_oaspl", "acum_tonal_spectra", "acum_tonal_fetch_spectra",
-           "acum_run_stage")
+           "acum_run_stage", "acum_run_save_cache")
 
 
 class AcumError(RuntimeError):
@@ -212,6 +213,10 @@ class Run:
 
     def save(self, path):
         _host_check(lib.acum_run_save(self._h, str(path).encode()))
+
+    def save_cache(self, path):
+        """Binary caches of the parsed surface files of run directory `path` (as read)."""
+        _host_check(lib.acum_run_save_cache(self._h, str(path).encode()))
 
     def tonal_case(self, precision=MIXED):
         c = TonalCase()

@@ -390,18 +390,73 @@ int read_observers(const std::string& path, acum_run* r) {
   return ACUM_OK;
 }
 
+// Binary cache of a parsed surface file: surfaces/surface_k.acumc beside surface_k.dat.  A
+// 64-byte header (magic, version, E, nt, and the size and modification time of the text file it
+// was made from) and then, element-major, 8 floats per (element, step): x y z A nx ny nz CP --
+// the run's own layout, so loading is a copy.  A cache whose header does not match the text
+// file's current size and time (or is short or damaged) is ignored and the text is parsed.
+constexpr char kCacheMagic[8] = {'A', 'C', 'U', 'M', 'S', 'R', 'F', '1'};
+struct CacheHeader {
+  char magic[8];
+  uint32_t version, floats_per_record;
+  int64_t E, nt, dat_size, dat_mtime_ns;
+  int64_t reserved[2];
+};
+static_assert(sizeof(CacheHeader) == 64, "cache header is 64 bytes");
+
+std::string cache_path(const std::string& dir, int64_t i) {
+  return join(dir, "surfaces/surface_" + std::to_string(i + 1) + ".acumc");
+}
+
+bool dat_stamp(const std::string& path, int64_t* size, int64_t* mtime_ns) {
+  struct stat st;
+  if (::stat(path.c_str(), &st) != 0) return false;
+  *size = (int64_t)st.st_size;
+  *mtime_ns = (int64_t)st.st_mtim.tv_sec * 1000000000LL + st.st_mtim.tv_nsec;
+  return true;
+}
+
+bool cache_disabled() {
+  const char* v = std::getenv("ACUM_NO_SURFACE_CACHE");
+  return v && v[0] && v[0] != '0';
+}
+
+// Maps surface i's cache when it matches the text file; the header is returned in *h.
+bool open_cache(const std::string& dir, int64_t i, MappedFile* m, CacheHeader* h) {
+  if (cache_disabled()) return false;
+  const std::string dat = join(dir, "surfaces/surface_" + std::to_string(i + 1) + ".dat");
+  int64_t size = 0, mt = 0;
+  if (!dat_stamp(dat, &size, &mt)) return false;
+  const std::string cp = cache_path(dir, i);
+  if (!exists(cp) || !m->open(cp) || m->size < sizeof(CacheHeader)) return false;
+  std::memcpy(h, m->data, sizeof(CacheHeader));
+  if (std::memcmp(h->magic, kCacheMagic, 8) != 0 || h->version != 1 || h->floats_per_record != 8 ||
+      h->E < 1 || h->nt < 1 || h->dat_size != size || h->dat_mtime_ns != mt)
+    return false;
+  const uint64_t need = sizeof(CacheHeader) + (uint64_t)h->E * (uint64_t)h->nt * 8 * sizeof(float);
+  return m->size == need;
+}
+
 // read_surf, io.cu:206-333 (impermeable branch): nTime blocks of E lines x y z CP A nx ny nz.
 // Each file is mapped, its numbers parsed in parallel into a time-major buffer, which is then
-// transposed in tiles into the element-major arrays.
+// transposed in tiles into the element-major arrays; a file with a matching binary cache is
+// copied from the cache instead.
 int read_surf(const std::string& dir, acum_run* r) {
   const int64_t nS = r->in.nSurf, nT = r->in.nTime;
   r->nSurf = nS;
   r->nTime = nT;
   r->E.assign(nS, 0);
-  std::vector<MappedFile> files(nS);
+  std::vector<MappedFile> files(nS), caches(nS);
   std::vector<size_t> body(nS, 0);
+  std::vector<char> cached(nS, 0);
   for (int64_t i = 0; i < nS; i++) {
     const std::string p = join(dir, "surfaces/surface_" + std::to_string(i + 1) + ".dat");
+    CacheHeader ch;
+    if (open_cache(dir, i, &caches[i], &ch) && ch.nt == nT) {
+      r->E[i] = ch.E;
+      cached[i] = 1;
+      continue;
+    }
     if (!files[i].open(p)) return fail(ACUM_ERR_IO, "missing surface file " + p);
     // header: the first two integers "E nt" (fscanf "%d %d\n", io.cu:235)
     const char* d = files[i].data;
@@ -429,6 +484,25 @@ int read_surf(const std::string& dir, acum_run* r) {
   fvec tmp((size_t)(nT * Emax * 8));  // time-major numbers of one file, first touched by the parse
   for (int64_t i = 0; i < nS; i++) {
     const int64_t E = r->E[i];
+    if (cached[i]) {
+      const float* c = (const float*)(caches[i].data + sizeof(CacheHeader));
+      float* X = r->X.data() + r->XPtr[i];
+      float* N = r->N.data() + r->NPtr[i];
+      float* P = r->PR.data() + r->PRPtr[i];
+#pragma omp parallel for schedule(static)
+      for (int64_t q = 0; q < E * nT; q++) {
+        const float* t = c + q * 8;
+        X[q * 4 + 0] = t[0];
+        X[q * 4 + 1] = t[1];
+        X[q * 4 + 2] = t[2];
+        X[q * 4 + 3] = t[3];
+        N[q * 6 + 0] = t[4];
+        N[q * 6 + 1] = t[5];
+        N[q * 6 + 2] = t[6];
+        P[q * 2] = t[7];
+      }
+      continue;
+    }
     const int64_t count = nT * E * 8;
     const char* b = files[i].data + body[i];
     const char* e = files[i].data + files[i].size;
@@ -855,6 +929,51 @@ int acum_run_write_bb(const acum_run* r, const char* dir, const float* BBoaspl, 
     for (const std::string& s : rows) std::fwrite(s.data(), 1, s.size(), f);
   }
   std::fclose(f);
+  return ACUM_OK;
+}
+
+int acum_run_save_cache(const acum_run* r, const char* dir) {
+  if (!r || !dir) return fail(ACUM_ERR_ARG, "null argument");
+  if (r->preprocessed) return fail(ACUM_ERR_ARG, "run already preprocessed; cache the run as read");
+  const std::string d = dir;
+  const int64_t nT = r->nTime;
+  for (int64_t i = 0; i < r->nSurf; i++) {
+    const std::string dat = join(d, "surfaces/surface_" + std::to_string(i + 1) + ".dat");
+    CacheHeader h{};
+    std::memcpy(h.magic, kCacheMagic, 8);
+    h.version = 1;
+    h.floats_per_record = 8;
+    h.E = r->E[i];
+    h.nt = nT;
+    if (!dat_stamp(dat, &h.dat_size, &h.dat_mtime_ns)) return fail(ACUM_ERR_IO, "missing surface file " + dat);
+    const int64_t n = h.E * nT;
+    std::vector<float> buf((size_t)n * 8);
+    const float* X = r->X.data() + r->XPtr[i];
+    const float* N = r->N.data() + r->NPtr[i];
+    const float* P = r->PR.data() + r->PRPtr[i];
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < n; q++) {
+      float* t = &buf[(size_t)q * 8];
+      t[0] = X[q * 4 + 0];
+      t[1] = X[q * 4 + 1];
+      t[2] = X[q * 4 + 2];
+      t[3] = X[q * 4 + 3];
+      t[4] = N[q * 6 + 0];
+      t[5] = N[q * 6 + 1];
+      t[6] = N[q * 6 + 2];
+      t[7] = P[q * 2];
+    }
+    // written beside the final name and renamed, so a reader never sees a partial cache
+    const std::string fin = cache_path(d, i), tmp = fin + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return fail(ACUM_ERR_IO, "cannot write " + tmp);
+    const bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 &&
+                    std::fwrite(buf.data(), sizeof(float), buf.size(), f) == buf.size();
+    if (std::fclose(f) != 0 || !ok || std::rename(tmp.c_str(), fin.c_str()) != 0) {
+      std::remove(tmp.c_str());
+      return fail(ACUM_ERR_IO, "cannot write " + fin);
+    }
+  }
   return ACUM_OK;
 }
 

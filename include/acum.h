@@ -187,6 +187,13 @@ int acum_run_write_tonal(const acum_run* run, const char* dir, const float* pT, 
 int acum_run_write_bb(const acum_run* run, const char* dir, const float* BBoaspl, const float* BBspl);
 /* Writes the input files of the run (input, obs.dat, surfaces/, input_BB, BB_inputs/).       */
 int acum_run_save(const acum_run* run, const char* dir);
+/* Binary cache of the parsed surface files: surfaces/surface_k.acumc beside each
+ * surface_k.dat (a 64-byte header with E, nt and the text file's size and modification time,
+ * then the parsed floats).  acum_run_load copies a matching cache instead of parsing the text;
+ * a cache that does not match the text file (or ACUM_NO_SURFACE_CACHE=1) is ignored.  No
+ * counterpart in the reference, which parses the text each run (io.cu:206-333).  The run must
+ * be as read (not preprocessed).                                                             */
+int acum_run_save_cache(const acum_run* run, const char* dir);
 /* Raw surface arrays as read (before preprocess): x,y,z,CP,A,nx,ny,nz per (surface, k, j).   */
 int acum_run_surface_raw(const acum_run* run, int32_t isurf, float* out /* [nTime*E*8] */);
 

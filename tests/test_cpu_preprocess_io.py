@@ -353,3 +353,45 @@ def test_tonal_writer_matches_printf(tmp_path):
             row = pf(np.float32(i) * dpsi) + "".join(pf(arr[i * nObs + j]) for j in range(nObs))
             lines.append(row)
         assert (tmp_path / name).read_text() == "\n".join(lines) + "\n", name
+
+
+def test_surface_cache_round_trip_and_invalidation(tmp_path):
+    """A run loaded through the binary surface caches equals the run parsed from the text; a
+    cache whose text file has changed, or a damaged cache, is ignored."""
+    import os
+    import time
+    d = tmp_path / "r"
+    p = acum.synth_params(1, nChord=3, nSpan=2, nTime=12)
+    acum.Run.synth(p).save(str(d))
+    text = acum.Run.load(str(d))
+    text.save_cache(str(d))
+    cache = d / "surfaces" / "surface_1.acumc"
+    assert cache.exists()
+    # parse from the cache: same arrays; make the text unreadable as numbers to prove it is not parsed
+    dat = d / "surfaces" / "surface_1.dat"
+    st = os.stat(dat)
+    raw = dat.read_bytes()
+    body = raw.split(b"\n", 1)
+    dat.write_bytes(body[0] + b"\n" + bytes(b if b in b" \n" else ord("x") for b in body[1]))
+    os.utime(dat, ns=(st.st_atime_ns, st.st_mtime_ns))
+    via = acum.Run.load(str(d))
+    for i in range(text.inputs().nSurf):
+        assert np.array_equal(text.surface_raw(i), via.surface_raw(i))
+    # the text changes (new time): the cache no longer matches and the (garbled) text is parsed
+    time.sleep(0.01)
+    os.utime(dat, None)
+    with pytest.raises(acum.AcumError):
+        acum.Run.load(str(d))
+    # restore the text: no cache matches it until one is written again
+    dat.write_bytes(raw)
+    again = acum.Run.load(str(d))
+    assert np.array_equal(text.surface_raw(0), again.surface_raw(0))
+    again.save_cache(str(d))
+    # a damaged cache is ignored
+    b = bytearray(cache.read_bytes())
+    b[0] ^= 0xFF
+    cache.write_bytes(bytes(b))
+    assert np.array_equal(text.surface_raw(0), acum.Run.load(str(d)).surface_raw(0))
+    # a preprocessed run cannot be cached
+    with pytest.raises(acum.AcumError):
+        acum.Run.load(str(d)).preprocess().save_cache(str(d))
