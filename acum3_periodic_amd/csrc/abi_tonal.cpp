@@ -932,7 +932,7 @@ int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t) {
   if (!ctx || !t || (precision != ACUM_FP64 && precision != ACUM_MIXED)) return ACUM_ERR_ARG;
   const bool mixed = precision == ACUM_MIXED;
   if (!eval_window_supported(mixed, t->window))
-    return ctx_fail(ctx, ACUM_ERR_ARG, "window must be 16, 24 or 32 (or 48, 64 for the mixed path)");
+    return ctx_fail(ctx, ACUM_ERR_ARG, "window must be 16, 20, 24 or 32 (or 48, 64 for the mixed path)");
   if (t->max_spread < 1 || t->max_tile < 1 || t->max_chunk < 1 || t->groups < 0)
     return ctx_fail(ctx, ACUM_ERR_ARG, "bad tile limits");
   ctx->tune[precision] = *t;

@@ -116,7 +116,8 @@ typedef struct acum_tune {
   int32_t max_tile;    /* most elements per tile                                              */
   int32_t groups;      /* partial slabs per observer row (0 = automatic)                      */
   int32_t max_chunk;   /* most source time steps per window pass                              */
-  int32_t waves;       /* waves per workgroup (observer lanes = 64 * waves)                   */
+  int32_t waves;       /* waves per workgroup: always 1 (one 64-observer wave per workgroup;  */
+                       /* the LDS window, not the workgroup size, sets the occupancy)         */
 } acum_tune;
 
 /* --- context ----------------------------------------------------------------------------- */
