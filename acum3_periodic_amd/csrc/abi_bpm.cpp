@@ -179,7 +179,9 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   HIPCHK(B.alloc(&d_tt, ntab));
   HIPCHK(B.alloc(&d_in, nsrc));
   HIPCHK(B.alloc(&d_src, nsrc));
-  HIPCHK(B.alloc(&d_S, (size_t)nsrc * nf2 * 6));
+  // the observer kernel reads a whole pass of bands per source, past nf2 on the last source
+  const size_t s_pad = (size_t)bpm_bands_per_pass(nf2) * 6;
+  HIPCHK(B.alloc(&d_S, (size_t)nsrc * nf2 * 6 + s_pad));
   HIPCHK(B.alloc(&d_ff, nf2));
   HIPCHK(B.alloc(&d_obs, (size_t)nObs * 3));
   HIPCHK(B.alloc(&d_part, (size_t)nchunk * nObs * nf2 * 6));
@@ -187,6 +189,7 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   HIPCHK(B.alloc(&d_oa, nObs));
   HIPCHK(hipMemcpyAsync(d_surf, surf.data(), nS * sizeof(BpmSurfDev), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemsetAsync(d_err, 0, sizeof(uint32_t), s));
+  HIPCHK(hipMemsetAsync(d_S + (size_t)nsrc * nf2 * 6, 0, s_pad * sizeof(float), s));
   HIPCHK(hipMemcpyAsync(d_x, c->xyzcR, nx * sizeof(float), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_bd, c->bbdata, nd * sizeof(float), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_st, c->sect_t, nst * sizeof(float), hipMemcpyHostToDevice, s));

@@ -215,12 +215,18 @@ __global__ void bpm_shape_kernel(const BpmShapeIn* __restrict__ in, int64_t nsrc
   dev[i].flags = (sep ? 1u : 0u) | (lam ? 2u : 0u) | (q.tip ? 4u : 0u);
 }
 
+// max of two non-negative floats as one integer max (their bit patterns order as the values do),
+// without the canonicalising maxes that an IEEE fmaxf of a loaded value costs
+__device__ __forceinline__ float max_nonneg(float a, float b) {
+  return __int_as_float(max(__float_as_int(a), __float_as_int(b)));
+}
+
 // One lane per observer, a chunk of sources per workgroup, NB bands from band b0.
 template <int NB>
 __global__ __launch_bounds__(64) void bpm_obs_kernel(const BpmSrcDev* __restrict__ src, const float* __restrict__ S,
                                                      int64_t nsrc, int64_t chunk, int nf2, int b0,
                                                      const float* __restrict__ obs, int nObs, float vx, float vy,
-                                                     float vz, float aMv, float* __restrict__ part) {
+                                                     float vz, float aMv, float inv_aMv, float* __restrict__ part) {
   const int o = blockIdx.x * 64 + threadIdx.x;
   const int ch = blockIdx.y;
   const int oc = o < nObs ? o : nObs - 1;
@@ -239,41 +245,44 @@ __global__ __launch_bounds__(64) void bpm_obs_kernel(const BpmSrcDev* __restrict
     float rx = ox - q.xt[0], ry = oy - q.xt[1], rz = oz - q.xt[2];
     const float bq = rx * vx + ry * vy + rz * vz;
     const float cq = rx * rx + ry * ry + rz * rz;
-    const float tau = (-bq + sqrtf(bq * bq + aMv * cq)) / aMv;
+    const float tau = (__builtin_amdgcn_sqrtf(bq * bq + aMv * cq) - bq) * inv_aMv;
     rx -= vx * tau;
     ry -= vy * tau;
     rz -= vz * tau;
     const float r2 = rx * rx + ry * ry + rz * rz;
-    const float ir2 = 1.0f / r2;
-    const float rn = sqrtf(r2);
+    const float ir2 = __builtin_amdgcn_rcpf(r2);
     const float p0 = rx * q.tr[0] + ry * q.tr[1] + rz * q.tr[2];
     const float p1 = rx * q.tr[3] + ry * q.tr[4] + rz * q.tr[5];
     const float pz = rx * q.tr[6] + ry * q.tr[7] + rz * q.tr[8];
-    const float re = sqrtf(p1 * p1 + pz * pz);
-    // theta = acos|p0/rn|, phi = acos(|p1|/re) (bb_core.cu:529-530) through their cosines
-    const float ct = fminf(fabsf(p0) / rn, 1.0f);
-    const float cph = fminf(fabsf(p1) / re, 1.0f);
+    // theta = acos|p0/r|, phi = acos(|p1|/re), re = |(p1, pz)| (bb_core.cu:529-530) through
+    // their cosines; re = 0 gives 0 * inf = NaN, which fminf turns into 1 as 0/0 did
+    const float ct = fminf(fabsf(p0) * __builtin_amdgcn_rsqf(r2), 1.0f);
+    const float cph = fminf(fabsf(p1) * __builtin_amdgcn_rsqf(p1 * p1 + pz * pz), 1.0f);
     const float sph2 = (1.0f - cph) * (1.0f + cph);
     const float sth2 = (1.0f - ct) * (1.0f + ct);
     const float d1 = 1.0f + q.M * ct, d2 = 1.0f + q.Mmc * ct;
-    const float Gh = (1.0f - ct) * sph2 / (d1 * d2 * d2) * ir2;  // D_h / r^2 (bb_core.cu:620)
+    const float Gh = (1.0f - ct) * sph2 * __builtin_amdgcn_rcpf(d1 * d2 * d2) * ir2;  // D_h / r^2 (bb_core.cu:620)
     const float dl1 = d1 * d1;
-    const float Gl = sth2 * sph2 / (dl1 * dl1) * ir2;            // D_l / r^2 (bb_core.cu:624)
+    const float Gl = sth2 * sph2 * __builtin_amdgcn_rcpf(dl1 * dl1) * ir2;            // D_l / r^2 (bb_core.cu:624)
     const float Ga = (q.flags & 1u) ? Gl : Gh;
-    const bool lam = (q.flags & 2u) != 0, tip = (q.flags & 4u) != 0;
     const float tipE = fmaxf(q.tipT0 * Gh, floor_t);
+    // max(G S, floor) = G max(S, floor / G) for G > 0: one max and one FMA per clipped
+    // component.  G is held at >= 1e-30, where floor / G stays finite and, with S below 1e20 (a
+    // 200 dB band), G S < floor already, so the clipped value is the floor as before.
+    const float Gae = fmaxf(Ga, 1.0e-30f), Ghe = fmaxf(Gh, 1.0e-30f);
+    const float fa = floor_e * __builtin_amdgcn_rcpf(Gae), fh = floor_e * __builtin_amdgcn_rcpf(Ghe);
+    // the LBL-VS and tip shapes are zero where their mechanism is off, so those components are
+    // added unconditionally; bands past nb read padding or the next source and are not stored
     const float* Sq = S + (s * nf2 + b0) * kComp;
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-      if (b < nb) {
-        const float* Sb = Sq + b * kComp;
-        acc[b][0] += fmaxf(Ga * Sb[0], floor_e);
-        acc[b][1] += fmaxf(Gh * Sb[1], floor_e);
-        acc[b][2] += fmaxf(Gh * Sb[2], floor_e);
-        if (lam) acc[b][3] += Gh * Sb[3];
-        acc[b][4] += Gh * Sb[4];
-        if (tip) acc[b][5] += tipE * Sb[5];
-      }
+      const float* Sb = Sq + b * kComp;
+      acc[b][0] = fmaf(Gae, max_nonneg(Sb[0], fa), acc[b][0]);
+      acc[b][1] = fmaf(Ghe, max_nonneg(Sb[1], fh), acc[b][1]);
+      acc[b][2] = fmaf(Ghe, max_nonneg(Sb[2], fh), acc[b][2]);
+      acc[b][3] = fmaf(Gh, Sb[3], acc[b][3]);
+      acc[b][4] = fmaf(Gh, Sb[4], acc[b][4]);
+      acc[b][5] = fmaf(tipE, Sb[5], acc[b][5]);
     }
   }
   if (o < nObs) {
@@ -348,18 +357,19 @@ int bpm_bands_per_pass(int nf2) {
 hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, int64_t nsrc, int64_t chunk,
                           int nchunk, int nf2, const float* obs, int nObs, float vx, float vy, float vz,
                           float aMv, float* part) {
+  const float inv_aMv = (float)(1.0 / (double)aMv);
   const int NB = bpm_bands_per_pass(nf2);
   for (int b0 = 0; b0 < nf2; b0 += NB) {
     dim3 g((nObs + 63) / 64, nchunk);
     if (NB == 17)
       hipLaunchKernelGGL(bpm_obs_kernel<17>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
-                         vz, aMv, part);
+                         vz, aMv, inv_aMv, part);
     else if (NB == 14)
       hipLaunchKernelGGL(bpm_obs_kernel<14>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
-                         vz, aMv, part);
+                         vz, aMv, inv_aMv, part);
     else
       hipLaunchKernelGGL(bpm_obs_kernel<12>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
-                         vz, aMv, part);
+                         vz, aMv, inv_aMv, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
