@@ -174,29 +174,38 @@ def summarize_sq(d):
     return 0
 
 
-def summarize_stats(d):
+def summarize_stats(d, workload="rotor"):
     """Kernel time table of one rocprofv3 --kernel-trace --stats run of
-    `bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2` (both tonal precisions)."""
+    `bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2` (both tonal precisions), or with
+    workload "bpm" of `bench.py --workload bpm --steps 3`."""
     import csv
     files = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)
     if not files:
         raise SystemExit(f"no kernel_stats.csv under {d}")
     rows = list(csv.DictReader(open(files[0])))
-    lines = ["# Kernel statistics: tonal path, four-blade rotor (4 x 20,000 elements, 1024 steps,",
-             "# 1024 observers), mixed and FP64 passes", "",
-             "Command (repository root `R` on the GPU host):", "", "```",
-             "cd /tmp && export TMPDIR=/tmp",
-             "rocprofv3 --kernel-trace --stats --output-format csv -d $R/gpurun_out/prof -o bench -- \\",
-             "    python3 $R/bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2",
-             "python3 $R/bench.py --summarize-stats $R/gpurun_out/prof", "```", "",
-             "`tonal_eval_kernel<true, ...>` is the mixed path, `<false, ...>` the FP64 path.", "",
-             "| kernel | calls | avg ms | min ms | max ms | share |", "|---|---|---|---|---|---|"]
+    if workload == "bpm":
+        head = ["# Kernel statistics: BPM broadband path, 4 x 60 sections, 360 steps, 1024 observers,",
+                "# 34 bands (one warm-up and three timed calls)"]
+        cmd = ["    python3 $R/bench.py --workload bpm --steps 3",
+               "python3 $R/bench.py --workload bpm --summarize-stats $R/gpurun_out/prof"]
+        note_ = ["`bpm_obs_kernel<17>` runs twice per call (34 bands in two passes of 17).", ""]
+        out = os.path.join(ROOT, "profiles", "bpm_c3_kernel_stats.md")
+    else:
+        head = ["# Kernel statistics: tonal path, four-blade rotor (4 x 20,000 elements, 1024 steps,",
+                "# 1024 observers), mixed and FP64 passes"]
+        cmd = ["    python3 $R/bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2",
+               "python3 $R/bench.py --summarize-stats $R/gpurun_out/prof"]
+        note_ = ["`tonal_eval_kernel<true, ...>` is the mixed path, `<false, ...>` the FP64 path.", ""]
+        out = os.path.join(ROOT, "profiles", "tonal_c2_kernel_stats.md")
+    lines = head + ["", "Command (repository root `R` on the GPU host):", "", "```",
+                    "cd /tmp && export TMPDIR=/tmp",
+                    "rocprofv3 --kernel-trace --stats --output-format csv -d $R/gpurun_out/prof -o bench -- \\"] + \
+        cmd + ["```", ""] + note_ + ["| kernel | calls | avg ms | min ms | max ms | share |", "|---|---|---|---|---|---|"]
     for r in rows:
-        n = r["Name"].split("(")[0].replace("void ", "").replace("acum::", "")
+        n = r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").replace("acum::", "")
         lines.append(f"| `{n}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.3f} | "
                      f"{float(r['MinNs']) / 1e6:.3f} | {float(r['MaxNs']) / 1e6:.3f} | "
                      f"{float(r['Percentage']):.2f}% |")
-    out = os.path.join(ROOT, "profiles", "tonal_c2_kernel_stats.md")
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -400,7 +409,7 @@ def main():
     ap.add_argument("--wave", type=int, default=8)
     args = ap.parse_args()
     if args.summarize_stats:
-        return summarize_stats(args.summarize_stats)
+        return summarize_stats(args.summarize_stats, args.workload)
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc)
     if args.summarize_sq:
