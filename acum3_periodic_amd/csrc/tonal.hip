@@ -4,14 +4,17 @@
 // float atomics on pT/pL/pA, core.cu:18-259, main.cu:194-236):
 //   * one wave = 64 observers (one per lane); element records are wave-uniform scalar loads, so
 //     every record read from HBM serves 64 observers;
-//   * a work item = (case, observer group, group of tiles); a tile is a run of elements that sit
-//     close together; for each tile the wave walks the source steps in passes of K steps and, for
-//     each element of the tile, the K steps of that pass in order -- one FP64 retarded solve per
-//     (element, step, observer) instead of the reference's three (core.cu:84,189,193);
+//   * a work item = (case, observer group, group of tiles), taken from one of eight queues (one
+//     per XCD, so the observer groups reading the same records share an L2); a tile is a run of
+//     elements that sit close together; for each tile the wave walks the source steps in passes
+//     of K steps and, for each element of the tile, the K steps of that pass in order -- one FP64
+//     retarded solve per (element, step, observer) instead of the reference's three
+//     (core.cu:84,189,193);
 //   * each lane owns a column of an LDS window of W observer-time bins; the tent-weighted
 //     contributions of an eval (core.cu:196-231) are read-modify-written into the 4-slot
-//     footprint ic-1 .. ic+2 around its top rising bin ic with predicated weights, rarer wider
-//     tents in a wave-uniform tail; no other lane touches the column, so the order is fixed;
+//     footprint ic-1 .. ic+2 around its top rising bin ic with weights clamped at zero (by the
+//     FMA's clamp bit), rarer wider tents in a wave-uniform tail; no other lane touches the
+//     column, so the order is fixed;
 //   * after each pass the window is added into the item's row of a partial slab in HBM;
 //   * a reduction kernel sums the partial slabs of each (observer, bin) in fixed order with
 //     FP64 Kahan compensation and writes pT, pL and pA = pT + pL.
