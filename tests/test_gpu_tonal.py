@@ -156,3 +156,22 @@ def test_errors_are_returned(ctx):
             assert e.value.code == acum.ACUM_ERR_NUMERIC
     run, c = small_case(1, acum.FP64)
     ctx.tonal(c)  # the context keeps working after those errors
+
+
+def test_c1_full_size(ctx):
+    """The reference's CPU-runnable configuration at its full size (2 surfaces x 1000 elements,
+    360 steps, 36 observers on an arc, hover): both precisions against the FP64 oracle, and the
+    bin indices of every source for the first and last observer."""
+    run = acum.Run.synth(acum.synth_params(1)).preprocess()
+    for prec, bar in ((acum.FP64, REL_L2_FP64), (acum.MIXED, REL_L2_MIXED)):
+        c = run.tonal_case(prec)
+        assert c.nTime == 360 and c.nObs == 36 and c.E[0] == 1000
+        g = ctx.tonal(c)
+        o = O.tonal_f64(c)
+        for x, y in zip(g, o):
+            assert O.rel_l2(x, y) <= bar
+        d = np.abs(acum.oaspl(g[2]) - acum.oaspl(o[2].astype(np.float32)))
+        assert d.max() <= OASPL_DB
+    c = run.tonal_case(acum.FP64)
+    for iobs in (0, c.nObs - 1):
+        assert all(np.array_equal(a, b) for a, b in zip(ctx.bins(c, iobs), O.tonal_bins(c, iobs)))
