@@ -416,7 +416,7 @@ def main():
         return summarize_sq(args.summarize_sq)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(observer sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(observers would be sharded across GPUs with no communication); run with --gpus 1", file=sys.stderr)
         return 2
 
     import acum3_periodic_amd as acum
