@@ -113,15 +113,15 @@ def summarize_pmc(fetch_dir, write_dir):
         "# HBM traffic of the tonal eval kernel: four-blade rotor (4 x 20,000 elements, 1024 steps,",
         "# 1024 observers), mixed path, one launch",
         "",
-        "Commands (repository root `R` on the GPU host, one counter per run):",
+        "Commands (repository root `R`, output directory `OUT`, on the GPU host, one counter per run):",
         "",
         "```",
         "cd /tmp && export TMPDIR=/tmp",
-        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d $R/gpurun_out/pmcF -o p -- \\",
+        "rocprofv3 --pmc FETCH_SIZE --output-format csv -d $OUT/pmcF -o p -- \\",
         "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1 --copy-calibration",
-        "rocprofv3 --pmc WRITE_SIZE --output-format csv -d $R/gpurun_out/pmcW -o p -- \\",
+        "rocprofv3 --pmc WRITE_SIZE --output-format csv -d $OUT/pmcW -o p -- \\",
         "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1 --copy-calibration",
-        "python3 $R/bench.py --summarize-pmc $R/gpurun_out/pmcF $R/gpurun_out/pmcW",
+        "python3 $R/bench.py --summarize-pmc $OUT/pmcF $OUT/pmcW",
         "```",
         "",
         f"Calibration: a device copy of {CALIB_BYTES} bytes (reads and writes that many bytes).",
@@ -156,11 +156,11 @@ def summarize_sq(d):
     wave_evals = 80000 * 1024 * 1024 / 64  # c2: elements x steps x observers / 64 lanes
     lines = ["# Instruction mix of the tonal eval kernel (mixed path): four-blade rotor, 4 x 20,000",
              "# elements, 1024 steps, 1024 observers, one launch", "",
-             "Command (repository root `R` on the GPU host):", "", "```",
+             "Command (repository root `R`, output directory `OUT`, on the GPU host):", "", "```",
              "cd /tmp && export TMPDIR=/tmp",
-             "rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d $R/gpurun_out/pmcSQ -o p -- \\",
+             "rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d $OUT/pmcSQ -o p -- \\",
              "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1",
-             "python3 $R/bench.py --summarize-sq $R/gpurun_out/pmcSQ", "```", "",
+             "python3 $R/bench.py --summarize-sq $OUT/pmcSQ", "```", "",
              f"Per wave-eval = counter / {wave_evals:.4g} (8.39e10 evals / 64 lanes). WAVE_CYCLES and",
              "WAIT_INST_ANY count in units of 4 cycles.", "",
              "| counter | total | per wave-eval |", "|---|---|---|"]
@@ -187,19 +187,19 @@ def summarize_stats(d, workload="rotor"):
         head = ["# Kernel statistics: BPM broadband path, 4 x 60 sections, 360 steps, 1024 observers,",
                 "# 34 bands (one warm-up and three timed calls)"]
         cmd = ["    python3 $R/bench.py --workload bpm --steps 3",
-               "python3 $R/bench.py --workload bpm --summarize-stats $R/gpurun_out/prof"]
+               "python3 $R/bench.py --workload bpm --summarize-stats $OUT/prof"]
         note_ = ["`bpm_obs_kernel<17>` runs twice per call (34 bands in two passes of 17).", ""]
         out = os.path.join(ROOT, "profiles", "bpm_c3_kernel_stats.md")
     else:
         head = ["# Kernel statistics: tonal path, four-blade rotor (4 x 20,000 elements, 1024 steps,",
                 "# 1024 observers), mixed and FP64 passes"]
         cmd = ["    python3 $R/bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2",
-               "python3 $R/bench.py --summarize-stats $R/gpurun_out/prof"]
+               "python3 $R/bench.py --summarize-stats $OUT/prof"]
         note_ = ["`tonal_eval_kernel<true, ...>` is the mixed path, `<false, ...>` the FP64 path.", ""]
         out = os.path.join(ROOT, "profiles", "tonal_c2_kernel_stats.md")
-    lines = head + ["", "Command (repository root `R` on the GPU host):", "", "```",
+    lines = head + ["", "Command (repository root `R`, output directory `OUT`, on the GPU host):", "", "```",
                     "cd /tmp && export TMPDIR=/tmp",
-                    "rocprofv3 --kernel-trace --stats --output-format csv -d $R/gpurun_out/prof -o bench -- \\"] + \
+                    "rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/prof -o bench -- \\"] + \
         cmd + ["```", ""] + note_ + ["| kernel | calls | avg ms | min ms | max ms | share |", "|---|---|---|---|---|---|"]
     for r in rows:
         n = r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").replace("acum::", "")
