@@ -33,6 +33,38 @@ extern "C" int probe_xcc(int* ids, int nblocks) {
   return (int)e;
 }
 
+// One wave of v_mfma_f32_4x4x1f32 (16 blocks of 4x4, K = 1), accumulated over three products:
+// d = a0 b0 + a1 b1 + a2 b2 with lane l supplying a_k[l] and b_k[l]; out[l][v] = d[v] of lane l.
+// The test checks the lane layout the tonal kernel relies on.
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ void mfma_kernel(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out) {
+  const int l = threadIdx.x;
+  v4f d = {0.0f, 0.0f, 0.0f, 0.0f};
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(a[l], b[l], d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(a[64 + l], b[64 + l], d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(a[128 + l], b[128 + l], d, 0, 0, 0);
+  for (int v = 0; v < 4; v++) out[4 * l + v] = d[v];
+}
+
+// a, b: host arrays of 3 x 64 floats; out: 64 x 4 floats.
+extern "C" int probe_mfma4x4(const float* a, const float* b, float* out) {
+  float *da = nullptr, *db = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&da, 192 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&db, 192 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dout, 256 * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(da, a, 192 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(db, b, 192 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(mfma_kernel, dim3(1), dim3(64), 0, 0, da, db, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 256 * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(da);
+  (void)hipFree(db);
+  (void)hipFree(dout);
+  return (int)e;
+}
+
 // x: host array of n positive doubles; s, r: host outputs (sqrt, reciprocal).  Returns 0 on
 // success, a HIP error code otherwise.
 extern "C" int probe_fp64(const double* x, double* s, double* r, int64_t n) {

@@ -47,3 +47,24 @@ def test_xcc_ids_spread_over_the_xcds():
     print("XCC_ID counts:", counts.tolist())
     assert ids.min() >= 0 and ids.max() < 8
     assert (counts[:8] > 0).sum() == 8
+
+
+def test_mfma_4x4x1_lane_layout():
+    """v_mfma_f32_4x4x1f32 as the tonal kernel uses it: 16 blocks of four lanes; lane l = 4b + j
+    supplies row j of block b's A and column j of its B, and receives column j of block b's
+    result in its four accumulators (one per row)."""
+    lib = C.CDLL(PROBE)
+    fp = C.POINTER(C.c_float)
+    lib.probe_mfma4x4.argtypes = [fp, fp, fp]
+    lib.probe_mfma4x4.restype = C.c_int
+    rng = np.random.default_rng(5)
+    a = rng.integers(-8, 9, size=(3, 64)).astype(np.float32)   # small integers: exact products
+    b = rng.integers(-8, 9, size=(3, 64)).astype(np.float32)
+    out = np.zeros((64, 4), np.float32)
+    assert lib.probe_mfma4x4(a.ctypes.data_as(fp), b.ctypes.data_as(fp), out.ctypes.data_as(fp)) == 0
+    want = np.zeros((64, 4), np.float32)
+    for l in range(64):
+        blk, j = l // 4, l % 4
+        for v in range(4):
+            want[l, v] = sum(a[k, 4 * blk + v] * b[k, 4 * blk + j] for k in range(3))
+    assert np.array_equal(out, want), f"layout differs:\n{out[:8]}\n{want[:8]}"
