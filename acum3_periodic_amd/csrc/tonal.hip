@@ -128,32 +128,85 @@ struct Prec<true> {
 // where r = aB T and r_vec = d - u T (core.cu:86; aB = a0 / KB, u = vinf / KB).  Returns r (1 - Mr):
 // positive for a valid eval, zero for a source on the observer (r = 0 gives r_vec = 0 and dV = 0)
 // and negative for Mr > 1.
+// r and r_vec of an eval
+template <typename R, bool MOVING>
+__device__ __forceinline__ void radiation_vector(R dx, R dy, R dz, R del, R aB, R vx, R vy, R vz,
+                                                 R* r, R* rx, R* ry, R* rz) {
+  if (MOVING) {
+    *r = aB * del;
+    *rx = fmaT(-vx, del, dx);
+    *ry = fmaT(-vy, del, dy);
+    *rz = fmaT(-vz, del, dz);
+  } else {
+    *r = del;
+    *rx = dx;
+    *ry = dy;
+    *rz = dz;
+  }
+}
+
+// the integrand from r and the four record products
+template <typename R>
+__device__ __forceinline__ R integrand_tail(R aM1, R aLM, R cT, R cL, R r, R dV, R dVd, R dL,
+                                            R dLd, R a0, R* pth, R* plo) {
+  const R rm = r - dV;                // r (1 - Mr)
+  const R g = rcpT(rm);
+  const R g2 = g * g;
+  const R w = dVd - aM1;
+  *pth = (cT * g2) * fmaT(g * r, w, -a0);
+  *plo = (cL * g2) * fmaT(dL * g, w, dLd - aLM);
+  return rm;
+}
+
 template <typename R, bool MOVING>
 __device__ __forceinline__ R integrand(const AttrT<R>& q, R dx, R dy, R dz, R del, R a0, R aB,
                                        R vx, R vy, R vz, R* pth, R* plo) {
   R r, rx, ry, rz;
-  if (MOVING) {
-    r = aB * del;
-    rx = fmaT(-vx, del, dx);
-    ry = fmaT(-vy, del, dy);
-    rz = fmaT(-vz, del, dz);
-  } else {
-    r = del;
-    rx = dx;
-    ry = dy;
-    rz = dz;
-  }
+  radiation_vector<R, MOVING>(dx, dy, dz, del, aB, vx, vy, vz, &r, &rx, &ry, &rz);
   const R dV = fmaT(q.v[2], rz, fmaT(q.v[1], ry, q.v[0] * rx));
   const R dVd = fmaT(q.vd[2], rz, fmaT(q.vd[1], ry, q.vd[0] * rx));
   const R dL = fmaT(q.l[2], rz, fmaT(q.l[1], ry, q.l[0] * rx));
   const R dLd = fmaT(q.ld[2], rz, fmaT(q.ld[1], ry, q.ld[0] * rx));
-  const R rm = r - dV;                // r (1 - Mr)
-  const R g = rcpT(rm);
-  const R g2 = g * g;
-  const R w = dVd - q.aM1;
-  *pth = (q.cT * g2) * fmaT(g * r, w, -a0);
-  *plo = (q.cL * g2) * fmaT(dL * g, w, dLd - q.aLM);
-  return rm;
+  return integrand_tail<R>(q.aM1, q.aLM, q.cT, q.cL, r, dV, dVd, dL, dLd, a0, pth, plo);
+}
+
+// Mixed path: the four record products of 64 observers by the matrix cores.  With
+// v_mfma_f32_4x4x1f32 (16 blocks of four lanes, K = 1) lane l = 4b + j supplies row j of block b's
+// A and column j of its B and receives column j of block b's result: so lane l supplies row
+// l % 4 of the 4 x 3 record matrix (v, vd, l, ld) for A, its observer's r_vec component for B,
+// and after three accumulating products (x, y, z) holds dV, dVd, dL, dLd of its own observer.
+// The record rows reach the lanes by one 12-byte vector load per lane from the attribute record
+// (rows at float offsets 0, 3, 6, 9); the scalar fields come through a 16-byte scalar load.
+typedef float v4f __attribute__((ext_vector_type(4)));
+struct RecRow {
+  float x, y, z;
+};
+struct RecTail {
+  float aM1, aLM, cT, cL;
+};
+struct MixRec {
+  RecTail t;
+  RecRow row;
+};
+__device__ __forceinline__ MixRec load_mixrec(const AttrT<float>* rec, int lane) {
+  static_assert(sizeof(AttrT<float>) == 64, "attribute record layout");
+  MixRec m;
+  m.t = load_words<RecTail>((kptr<RecTail>)(uintptr_t)((const char*)rec + 48));
+  m.row = load_words<RecRow>((cptr<RecRow>)(uintptr_t)((const float*)rec + 3 * (lane & 3)));
+  return m;
+}
+
+template <bool MOVING>
+__device__ __forceinline__ float integrand_mfma(const MixRec& q, float dx, float dy, float dz, float del,
+                                                float a0, float aB, float vx, float vy, float vz,
+                                                float* pth, float* plo) {
+  float r, rx, ry, rz;
+  radiation_vector<float, MOVING>(dx, dy, dz, del, aB, vx, vy, vz, &r, &rx, &ry, &rz);
+  v4f d = {0.0f, 0.0f, 0.0f, 0.0f};
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.x, rx, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.y, ry, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.z, rz, d, 0, 0, 0);
+  return integrand_tail<float>(q.t.aM1, q.t.aLM, q.t.cT, q.t.cL, r, d[0], d[1], d[2], d[3], a0, pth, plo);
 }
 
 // The high word of a finite value read as a signed integer is positive exactly when the value is
@@ -186,6 +239,9 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
   using R = typename P::R;
   using R2 = typename P::R2;
   using Attr = AttrT<R>;
+  // what a step reads of its attribute record: the whole record (FP64), or the scalar fields
+  // and the lane's row for the matrix-core products (mixed)
+  using Rec = std::conditional_t<MIXED, MixRec, Attr>;
   using PosPtr = typename P::template ptr<PosRec>;
   using AttrPtr = typename P::template ptr<Attr>;
   // win[slot][lane] = (thickness, loading) partial of one observer-time bin
@@ -279,13 +335,19 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
           // record pipeline: positions one step ahead of the solve, attributes one step ahead
           // of the integrand, each with one load in flight
           PosRec p0 = load_words<PosRec>(pe + kn0), p1;
-          Attr a0r = load_words<Attr>(ae + k0), a1r;
+          auto load_rec = [&](int kk) __attribute__((always_inline)) -> Rec {
+            if constexpr (MIXED)
+              return load_mixrec((const Attr*)(uintptr_t)(ae + kk), lane);
+            else
+              return load_words<Attr>(ae + kk);
+          };
+          Rec a0r = load_rec(k0), a1r;
           auto step = [&](int kk, bool wrap, Step& cur, Step& prv, const PosRec& pn, PosRec& pnn,
-                          const Attr& ac, Attr& an) __attribute__((always_inline)) {
+                          const Rec& ac, Rec& an) __attribute__((always_inline)) {
             const int kn = (kk + 1 < nT) ? kk + 1 : 0;
             const int knn = (kn + 1 < nT) ? kn + 1 : 0;
             pnn = load_words<PosRec>(pe + knn);
-            an = load_words<Attr>(ae + kn);
+            an = load_rec(kn);
             // the footprint is known before the solve: start its window reads now
             const int ic = cur.ic;
             const int sb = min(max(ic - 1 - base, 0), W - 4);
@@ -301,7 +363,11 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
             if (wrap) t_next = t_next + dnT;
             // integrand of this step
             R pth, plo;
-            const R rm = integrand<R, MOVING>(ac, dxc, dyc, dzc, delc, a0, aB, ux, uy, uz, &pth, &plo);
+            R rm;
+            if constexpr (MIXED)
+              rm = integrand_mfma<MOVING>(ac, dxc, dyc, dzc, delc, a0, aB, ux, uy, uz, &pth, &plo);
+            else
+              rm = integrand<R, MOVING>(ac, dxc, dyc, dzc, delc, a0, aB, ux, uy, uz, &pth, &plo);
             // tent weights of the footprint
             const R fr = fractT(cur.t, R());  // tau - ic, exact
             const R dn = (R)(t_next - cur.t);
@@ -378,7 +444,7 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
             const PosRec tp = p0;
             p0 = p1;
             p1 = tp;
-            const Attr ta = a0r;
+            const Rec ta = a0r;
             a0r = a1r;
             a1r = ta;
           }
