@@ -3,8 +3,11 @@
 // preprocesses on the host, runs the tonal (and broadband) paths on the GPU through the C-ABI and
 // writes pT.out, pL.out, pA.out (and bb_oaspl.out, bb_spl.dat) into the same directory.
 //
-//   acum [run_dir] [--fp64 | --mixed] [--device N]
+//   acum [run_dir] [--fp64 | --mixed] [--device N] [--gpu-prep] [--cache]
 //   acum --generate CONFIG OUT_DIR      (writes a synthetic case: CONFIG 1, 2, 3 or 5)
+// --gpu-prep preprocesses on the GPU in place (acum_run_stage: the element data go to the device
+// once and never come back); --cache writes binary caches of the parsed surface files, which
+// later runs read instead of the text.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -21,11 +24,14 @@ static double now_ms() {
 int main(int argc, char** argv) {
   std::string dir = ".";
   int precision = ACUM_MIXED, device = 0;
+  bool gpu_prep = false, cache = false;
   for (int i = 1; i < argc; i++) {
     const std::string a = argv[i];
     if (a == "--fp64") precision = ACUM_FP64;
     else if (a == "--mixed") precision = ACUM_MIXED;
     else if (a == "--device" && i + 1 < argc) device = std::atoi(argv[++i]);
+    else if (a == "--gpu-prep") gpu_prep = true;
+    else if (a == "--cache") cache = true;
     else if (a == "--generate" && i + 2 < argc) {
       acum_synth_params p;
       if (acum_synth_defaults(std::atoi(argv[i + 1]), &p)) {
@@ -40,7 +46,8 @@ int main(int argc, char** argv) {
       acum_run_free(r);
       return 0;
     } else if (a == "-h" || a == "--help") {
-      std::printf("usage: acum [run_dir] [--fp64|--mixed] [--device N]\n       acum --generate CONFIG OUT_DIR\n");
+      std::printf("usage: acum [run_dir] [--fp64|--mixed] [--device N] [--gpu-prep] [--cache]\n"
+                  "       acum --generate CONFIG OUT_DIR\n");
       return 0;
     } else dir = a;
   }
@@ -51,7 +58,11 @@ int main(int argc, char** argv) {
     return 1;
   }
   double t1 = now_ms();
-  if (acum_run_preprocess(run)) {
+  if (cache && acum_run_save_cache(run, dir.c_str())) {
+    std::fprintf(stderr, "acum: %s\n", acum_host_error());
+    return 1;
+  }
+  if (!gpu_prep && acum_run_preprocess(run)) {
     std::fprintf(stderr, "acum: %s\n", acum_host_error());
     return 1;
   }
@@ -64,10 +75,19 @@ int main(int argc, char** argv) {
     return 1;
   }
   acum_tonal_case tc;
-  acum_run_tonal_case(run, precision, &tc);
+  (void)acum_run_tonal_case(run, precision, &tc);  // sizes; not preprocessed yet with --gpu-prep
   const size_t n = (size_t)tc.nTime * tc.nObs;
   std::vector<float> pT(n), pL(n), pA(n);
-  int rc = acum_tonal(ctx, &tc, pT.data(), pL.data(), pA.data());
+  int rc;
+  if (gpu_prep) {
+    int32_t h = 0;
+    rc = acum_run_stage(ctx, run, &h);
+    if (!rc) rc = acum_tonal_run(ctx, h, precision, 1, 0);
+    if (!rc) rc = acum_tonal_fetch(ctx, h, 0, pT.data(), pL.data(), pA.data());
+    if (!rc) rc = acum_tonal_release(ctx, h);
+  } else {
+    rc = acum_tonal(ctx, &tc, pT.data(), pL.data(), pA.data());
+  }
   if (rc) {
     std::fprintf(stderr, "acum: tonal: %s\n", acum_last_error(ctx));
     return 1;

@@ -181,6 +181,9 @@ int acum_preprocess_gpu(acum_ctx* ctx, acum_run* run);
  * Replaces the host preprocess_imper call (main.cu:88) and the device copies (main.cu:156-234). */
 int acum_run_stage(acum_ctx* ctx, const acum_run* run, int32_t* handle);
 int acum_run_inputs(const acum_run* run, acum_inputs* in);
+/* The run's tonal case (pointers into the run).  *c is filled in either way, so its sizes can be
+ * read from a run that is not preprocessed, but then the call returns ACUM_ERR_ARG: its element
+ * arrays are not yet the kernel's inputs (preprocess first, or stage with acum_run_stage).    */
 int acum_run_tonal_case(const acum_run* run, int32_t precision, acum_tonal_case* c);
 int acum_run_bpm_case(const acum_run* run, acum_bpm_case* c);
 /* pT.out, pL.out, pA.out (io.cu:137-203) and bb_oaspl.out, bb_spl.dat (io.cu:702-796).      */
