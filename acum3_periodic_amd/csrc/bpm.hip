@@ -296,44 +296,63 @@ __global__ __launch_bounds__(64) void bpm_obs_kernel(const BpmSrcDev* __restrict
 }
 
 // Ordered FP64 sums over chunks, time average and overall level (bb_core.cu:565-583).  One
-// workgroup per observer, one thread per band: each thread sums its band's six components over
-// the chunks in chunk order (Kahan), then thread 0 adds the band totals in band order.
-constexpr int kFinalThreads = 64;
+// workgroup per observer, one thread per (band, component) of up to kFinalThreads / 6 bands at a
+// time: each thread sums its cell over the chunks in chunk order (Kahan; a chunk's cells are
+// contiguous, so a workgroup's loads are too, and eight chunks' loads are issued before their
+// sums); a thread per band then adds its six sums in component order, and thread 0 adds the band
+// totals in band order.
+constexpr int kFinalThreads = 1024;
+constexpr int kFinalBatch = 8;
 
 __global__ __launch_bounds__(kFinalThreads) void bpm_final_kernel(const float* __restrict__ part,
                                                                   int nchunk, int nObs, int nf2, int nT,
                                                                   float* __restrict__ spl,
                                                                   float* __restrict__ oaspl) {
   const int o = blockIdx.x;
-  __shared__ double band_db[kFinalThreads];
+  const int bpi = blockDim.x / kComp;  // bands per iteration
+  __shared__ double sums[kFinalThreads];
+  __shared__ double band_db[kFinalThreads / kComp];
   const double flo = 1.0e-10 * (1.0 + (double)nT);
+  const int64_t cstride = (int64_t)nObs * nf2 * kComp;  // floats between chunks
   double oa = 1.0e-10;
-  for (int bb = 0; bb < nf2; bb += kFinalThreads) {
-    const int b = bb + threadIdx.x;
-    if (b < nf2) {
-      double sum[kComp], cmp[kComp];
-      for (int c = 0; c < kComp; c++) sum[c] = cmp[c] = 0.0;
-      for (int ch = 0; ch < nchunk; ch++) {
-        const float* p = part + (((int64_t)ch * nObs + o) * nf2 + b) * kComp;
-        for (int c = 0; c < kComp; c++) {
-          const double y = (double)p[c] - cmp[c], t = sum[c] + y;
-          cmp[c] = (t - sum[c]) - y;
-          sum[c] = t;
+  for (int bb = 0; bb < nf2; bb += bpi) {
+    const int nbi = min(bpi, nf2 - bb);
+    const int j = threadIdx.x;  // cell (band bb + j / 6, component j % 6)
+    if (j < nbi * kComp) {
+      const float* p = part + ((int64_t)o * nf2 + bb) * kComp + j;
+      double sum = 0.0, cmp = 0.0;
+      int ch = 0;
+      for (; ch + kFinalBatch <= nchunk; ch += kFinalBatch) {
+        float v[kFinalBatch];
+#pragma unroll
+        for (int u = 0; u < kFinalBatch; u++) v[u] = p[(int64_t)(ch + u) * cstride];
+#pragma unroll
+        for (int u = 0; u < kFinalBatch; u++) {
+          const double y = (double)v[u] - cmp, t = sum + y;
+          cmp = (t - sum) - y;
+          sum = t;
         }
       }
-      double tot = flo;
-      for (int c = 0; c < kComp; c++) tot += sum[c];
-      float* out = spl + ((int64_t)o * nf2 + b) * 7;
-      const double t0 = 10.0 * log10(tot / nT);
-      out[0] = (float)t0;
-      for (int c = 0; c < kComp; c++) out[1 + c] = (float)(10.0 * log10((flo + sum[c]) / nT));
-      band_db[threadIdx.x] = t0;
+      for (; ch < nchunk; ch++) {
+        const double y = (double)p[(int64_t)ch * cstride] - cmp, t = sum + y;
+        cmp = (t - sum) - y;
+        sum = t;
+      }
+      sums[j] = sum;
+      spl[((int64_t)o * nf2 + bb + j / kComp) * 7 + 1 + j % kComp] = (float)(10.0 * log10((flo + sum) / nT));
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      const int n = min(kFinalThreads, nf2 - bb);
-      for (int k = 0; k < n; k++) oa += exp10(0.1 * band_db[k]);
+    if (threadIdx.x < nbi) {
+      const int bl = threadIdx.x;
+      double tot = flo;
+      for (int c = 0; c < kComp; c++) tot += sums[bl * kComp + c];
+      const double t0 = 10.0 * log10(tot / nT);
+      spl[((int64_t)o * nf2 + bb + bl) * 7] = (float)t0;
+      band_db[bl] = t0;
     }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int k = 0; k < nbi; k++) oa += exp10(0.1 * band_db[k]);
     __syncthreads();
   }
   if (threadIdx.x == 0) oaspl[o] = (float)(10.0 * log10(oa));
@@ -378,8 +397,11 @@ hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, i
 
 hipError_t launch_bpm_final(hipStream_t s, const float* part, int nchunk, int nObs, int nf2, int nT, float* spl,
                             float* oaspl) {
-  hipLaunchKernelGGL(bpm_final_kernel, dim3(nObs), dim3(kFinalThreads), 0, s, part, nchunk, nObs, nf2, nT,
-                     spl, oaspl);
+  // six cells per band, rounded up to whole waves
+  const int64_t cells = ((int64_t)nf2 * kComp + 63) / 64 * 64;
+  const int threads = cells < kFinalThreads ? (int)cells : kFinalThreads;
+  hipLaunchKernelGGL(bpm_final_kernel, dim3(nObs), dim3(threads), 0, s, part, nchunk, nObs, nf2, nT, spl,
+                     oaspl);
   return hipGetLastError();
 }
 
