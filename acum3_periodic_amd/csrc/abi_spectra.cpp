@@ -77,7 +77,7 @@ int spectra_device(acum_ctx* ctx, const float* d_pA, int64_t nT, int64_t nObs, d
       return ctx_fail(ctx, e_ == hipErrorOutOfMemory ? ACUM_ERR_NOMEM : ACUM_ERR_DEVICE,      \
                       std::string(#call) + ": " + hipGetErrorString(e_));                     \
   } while (0)
-  CHK(B.alloc(&d_E, (size_t)nbins * nObs));
+  CHK(B.alloc(&d_E, (size_t)(nbins + 1) * nObs));  // bin energies, then the observer means
   CHK(B.alloc(&d_lo, nBands));
   CHK(B.alloc(&d_hi, nBands));
   CHK(B.alloc(&d_spl, (size_t)nObs * nBands));
@@ -98,7 +98,7 @@ int spectra_device(acum_ctx* ctx, const float* d_pA, int64_t nT, int64_t nObs, d
   t->eval_ms = ms;
   t->total_ms = ms;
   t->evals = (int64_t)nbins * nT * nObs;
-  t->launches = 2;
+  t->launches = 3;
   return ACUM_OK;
 }
 

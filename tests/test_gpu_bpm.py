@@ -94,3 +94,20 @@ def test_bpm_table_errors_are_returned(ctx):
     assert e.value.code == acum.ACUM_ERR_ARG
     aoa[:] = keep
     ctx.bpm(c)
+
+
+def test_bpm_many_bands(ctx):
+    """180 log-spaced bands from 20 Hz to 20 kHz: more bands than one workgroup of the final sums
+    holds at once (170) and eleven passes of the observer kernel."""
+    import ctypes as C
+    run = acum.Run.synth(acum.synth_params(3, nSect=6, nTime=12, nObsAz=3, nObsEl=2, trip=0))
+    c = run.bpm_case()
+    ff = (20.0 * 1000.0 ** (np.arange(180) / 179.0)).astype(np.float32)
+    c.ff = ff.ctypes.data_as(C.POINTER(C.c_float))
+    c.nf2 = ff.size
+    c._ff = ff  # keeps the band array alive with the case
+    gspl, goa = ctx.bpm(c)
+    ospl, ooa = O.bpm_f64(c)
+    assert gspl.shape == (c.nObs, 180, 7)
+    assert np.abs(gspl.astype(np.float64) - ospl).max() <= SPL_DB
+    assert np.abs(goa - ooa).max() <= SPL_DB
