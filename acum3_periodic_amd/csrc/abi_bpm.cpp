@@ -40,27 +40,11 @@ hipError_t launch_bpm_final(hipStream_t s, const float* part, int nchunk, int nO
                             float* oaspl);
 int bpm_occupancy_blocks(int nf2);
 int bpm_bands_per_pass(int nf2);
-void bpm_release(acum_ctx*) {}
 }  // namespace acum
 
 using namespace acum;
 
 namespace {
-
-struct DevBufs {
-  std::vector<void*> p;
-  ~DevBufs() {
-    for (void* q : p) (void)hipFree(q);
-  }
-  template <typename T>
-  hipError_t alloc(T** out, size_t n) {
-    void* q = nullptr;
-    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
-    if (e == hipSuccess) p.push_back(q);
-    *out = (T*)q;
-    return e;
-  }
-};
 
 int validate(acum_ctx* ctx, const acum_bpm_case* c) {
   if (!c) return ctx_fail(ctx, ACUM_ERR_ARG, "null case");
@@ -161,32 +145,35 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   nchunk = std::min<int64_t>(nchunk, nsrc);
   const int64_t chunk = (nsrc + nchunk - 1) / nchunk;
   nchunk = (nsrc + chunk - 1) / chunk;
-  DevBufs B;
   BpmSurfDev* d_surf;
   BpmShapeIn* d_in;
   BpmSrcDev* d_src;
   uint32_t* d_err;
   float *d_x, *d_bd, *d_st, *d_mt, *d_rt, *d_at, *d_tt;
   float *d_S, *d_ff, *d_obs, *d_part, *d_spl, *d_oa;
-  HIPCHK(B.alloc(&d_surf, nS));
-  HIPCHK(B.alloc(&d_err, 1));
-  HIPCHK(B.alloc(&d_x, nx));
-  HIPCHK(B.alloc(&d_bd, nd));
-  HIPCHK(B.alloc(&d_st, nst));
-  HIPCHK(B.alloc(&d_mt, nm));
-  HIPCHK(B.alloc(&d_rt, nr));
-  HIPCHK(B.alloc(&d_at, na));
-  HIPCHK(B.alloc(&d_tt, ntab));
-  HIPCHK(B.alloc(&d_in, nsrc));
-  HIPCHK(B.alloc(&d_src, nsrc));
   // the observer kernel reads a whole pass of bands per source, past nf2 on the last source
   const size_t s_pad = (size_t)bpm_bands_per_pass(nf2) * 6;
-  HIPCHK(B.alloc(&d_S, (size_t)nsrc * nf2 * 6 + s_pad));
-  HIPCHK(B.alloc(&d_ff, nf2));
-  HIPCHK(B.alloc(&d_obs, (size_t)nObs * 3));
-  HIPCHK(B.alloc(&d_part, (size_t)nchunk * nObs * nf2 * 6));
-  HIPCHK(B.alloc(&d_spl, (size_t)nObs * nf2 * 7));
-  HIPCHK(B.alloc(&d_oa, nObs));
+  WorkspaceLayout L;
+  L.add(&d_surf, nS);
+  L.add(&d_err, 1);
+  L.add(&d_x, nx);
+  L.add(&d_bd, nd);
+  L.add(&d_st, nst);
+  L.add(&d_mt, nm);
+  L.add(&d_rt, nr);
+  L.add(&d_at, na);
+  L.add(&d_tt, ntab);
+  L.add(&d_in, nsrc);
+  L.add(&d_src, nsrc);
+  L.add(&d_S, (size_t)nsrc * nf2 * 6 + s_pad);
+  L.add(&d_ff, nf2);
+  L.add(&d_obs, (size_t)nObs * 3);
+  L.add(&d_part, (size_t)nchunk * nObs * nf2 * 6);
+  L.add(&d_spl, (size_t)nObs * nf2 * 7);
+  L.add(&d_oa, nObs);
+  void* ws = nullptr;
+  HIPCHK(ctx_workspace(ctx, L.bytes(), &ws));
+  L.assign(ws);
   HIPCHK(hipMemcpyAsync(d_surf, surf.data(), nS * sizeof(BpmSurfDev), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemsetAsync(d_err, 0, sizeof(uint32_t), s));
   HIPCHK(hipMemsetAsync(d_S + (size_t)nsrc * nf2 * 6, 0, s_pad * sizeof(float), s));

@@ -11,7 +11,37 @@ hipStream_t ctx_stream(acum_ctx* c);
 hipEvent_t ctx_event(acum_ctx* c, int i);
 acum_timing* ctx_timing(acum_ctx* c);
 int ctx_device(acum_ctx* c);
-void bpm_release(acum_ctx* c);
+// A device workspace owned by the context for the buffers of one call (broadband, spectra):
+// grown on demand, kept until the context is destroyed.  Calls on one context are serialised
+// and each synchronises its stream before returning, so one call's buffers never outlive it.
+hipError_t ctx_workspace(acum_ctx* c, size_t bytes, void** out);
+
+// Lays out a call's device buffers (at most 32) in one workspace, each at a 256-byte boundary.
+struct WorkspaceLayout {
+  struct Req {
+    void** ptr;
+    size_t bytes;
+  };
+  Req req[32];
+  int n = 0;
+  template <typename T>
+  void add(T** p, size_t count) {
+    req[n++] = Req{(void**)p, (count ? count : 1) * sizeof(T)};
+  }
+  static size_t round(size_t b) { return (b + 255) / 256 * 256; }
+  size_t bytes() const {
+    size_t b = 0;
+    for (int i = 0; i < n; i++) b += round(req[i].bytes);
+    return b;
+  }
+  void assign(void* base) const {
+    char* q = (char*)base;
+    for (int i = 0; i < n; i++) {
+      *req[i].ptr = q;
+      q += round(req[i].bytes);
+    }
+  }
+};
 // band spectra of a device-resident pA [nT][nObs] into host outputs (abi_spectra.cpp)
 int spectra_device(acum_ctx* ctx, const float* d_pA, int64_t nT, int64_t nObs, double dt, int nBands,
                    const float* fc, float* spl, float* oaspl);

@@ -14,21 +14,6 @@ namespace {
 constexpr double kSixthDown = 0.8908987181403393;  // 2^(-1/6)
 constexpr double kSixthUp = 1.122462048309373;     // 2^(1/6)
 
-struct Bufs {
-  std::vector<void*> p;
-  ~Bufs() {
-    for (void* q : p) (void)hipFree(q);
-  }
-  template <typename T>
-  hipError_t alloc(T** out, size_t n) {
-    void* q = nullptr;
-    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
-    if (e == hipSuccess) p.push_back(q);
-    *out = (T*)q;
-    return e;
-  }
-};
-
 }  // namespace
 
 // Bin range [lo_b, hi_b) of every band: bin k (frequency k df, df = 1 / (nT dt)) belongs to band
@@ -66,7 +51,6 @@ int spectra_device(acum_ctx* ctx, const float* d_pA, int64_t nT, int64_t nObs, d
   spectra_band_bins(nT, dt, nBands, fc, lo, hi);
   const int nbins = (int)(nT / 2);
   hipStream_t s = ctx_stream(ctx);
-  Bufs B;
   double* d_E;
   int *d_lo, *d_hi;
   float *d_spl, *d_oa;
@@ -77,11 +61,15 @@ int spectra_device(acum_ctx* ctx, const float* d_pA, int64_t nT, int64_t nObs, d
       return ctx_fail(ctx, e_ == hipErrorOutOfMemory ? ACUM_ERR_NOMEM : ACUM_ERR_DEVICE,      \
                       std::string(#call) + ": " + hipGetErrorString(e_));                     \
   } while (0)
-  CHK(B.alloc(&d_E, (size_t)(nbins + 1) * nObs));  // bin energies, then the observer means
-  CHK(B.alloc(&d_lo, nBands));
-  CHK(B.alloc(&d_hi, nBands));
-  CHK(B.alloc(&d_spl, (size_t)nObs * nBands));
-  CHK(B.alloc(&d_oa, nObs));
+  WorkspaceLayout L;
+  L.add(&d_E, (size_t)(nbins + 1) * nObs);  // bin energies, then the observer means
+  L.add(&d_lo, nBands);
+  L.add(&d_hi, nBands);
+  L.add(&d_spl, (size_t)nObs * nBands);
+  L.add(&d_oa, nObs);
+  void* ws = nullptr;
+  CHK(ctx_workspace(ctx, L.bytes(), &ws));
+  L.assign(ws);
   CHK(hipMemcpyAsync(d_lo, lo.data(), nBands * sizeof(int), hipMemcpyHostToDevice, s));
   CHK(hipMemcpyAsync(d_hi, hi.data(), nBands * sizeof(int), hipMemcpyHostToDevice, s));
   CHK(hipEventRecord(ctx_event(ctx, 0), s));

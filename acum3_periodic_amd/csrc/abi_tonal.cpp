@@ -93,6 +93,8 @@ struct acum_ctx {
   int cu_count = 0;
   std::map<int, std::unique_ptr<Stage>> stages;
   int next_handle = 1;
+  void* ws = nullptr;  // per-call workspace (ctx_workspace)
+  size_t ws_bytes = 0;
 };
 
 namespace acum {
@@ -106,6 +108,26 @@ hipStream_t ctx_stream(acum_ctx* c) { return c->stream; }
 hipEvent_t ctx_event(acum_ctx* c, int i) { return c->ev[i]; }
 acum_timing* ctx_timing(acum_ctx* c) { return &c->timing; }
 int ctx_device(acum_ctx* c) { return c->device; }
+
+hipError_t ctx_workspace(acum_ctx* c, size_t bytes, void** out) {
+  if (bytes > c->ws_bytes) {
+    void* old = c->ws;
+    c->ws = nullptr;
+    c->ws_bytes = 0;
+    if (old) {
+      const hipError_t e = hipFree(old);
+      if (e != hipSuccess) return e;
+    }
+    const hipError_t e = hipMalloc(&c->ws, bytes);
+    if (e != hipSuccess) {
+      c->ws = nullptr;
+      return e;
+    }
+    c->ws_bytes = bytes;
+  }
+  *out = c->ws;
+  return hipSuccess;
+}
 
 }  // namespace acum
 
@@ -770,7 +792,7 @@ void acum_ctx_destroy(acum_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   for (auto& kv : c->stages) free_stage(*kv.second);
-  acum::bpm_release(c);
+  (void)hipFree(c->ws);
   (void)hipFree(c->d_counter);
   (void)hipFree(c->d_flags);
   for (auto& e : c->ev)

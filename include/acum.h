@@ -121,6 +121,9 @@ typedef struct acum_tune {
 } acum_tune;
 
 /* --- context ----------------------------------------------------------------------------- */
+/* A context owns a stream on its device, the staged cases and one device workspace that the
+ * broadband and spectra calls carve their buffers from; the workspace grows to the largest such
+ * call and is freed by acum_ctx_destroy.                                                      */
 int acum_device_count(void);
 int acum_ctx_create(acum_ctx** ctx, int device);
 void acum_ctx_destroy(acum_ctx* ctx);
