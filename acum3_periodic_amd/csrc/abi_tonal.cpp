@@ -658,7 +658,14 @@ Stage* get_stage(acum_ctx* ctx, int32_t handle) {
 int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
   if (prec != ACUM_FP64 && prec != ACUM_MIXED) return ctx_fail(ctx, ACUM_ERR_ARG, "bad precision");
   if (repeats < 1) repeats = 1;
-  hipEvent_t e0, e1;
+  struct Events {  // destroyed on every return
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  hipEvent_t &e0 = ev.e[0], &e1 = ev.e[1];
   HIPCHK(ctx, hipEventCreate(&e0));
   HIPCHK(ctx, hipEventCreate(&e1));
   HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
@@ -673,8 +680,6 @@ int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   float prep = 0;
   HIPCHK(ctx, hipEventElapsedTime(&prep, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   double eval = 0, red = 0, total = 0;
   if (use_graph) {
     if (!p.exec) {
@@ -682,8 +687,11 @@ int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
       rc = enqueue_pass(ctx, st, prec, false);
       hipGraph_t g = nullptr;
       hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
-      if (rc) return rc;
-      HIPCHK(ctx, ce);
+      if (rc || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        HIPCHK(ctx, ce);
+      }
       p.graph = g;
       HIPCHK(ctx, hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
     }
@@ -763,22 +771,22 @@ int acum_ctx_create(acum_ctx** out, int device) {
   acum_ctx* c = new acum_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
+    acum_ctx_destroy(c);  // releases what was created so far
     return ACUM_ERR_DEVICE;
   }
   for (auto& e : c->ev)
     if (hipEventCreate(&e) != hipSuccess) {
-      delete c;
+      acum_ctx_destroy(c);
       return ACUM_ERR_DEVICE;
     }
   if (hipMalloc(&c->d_counter, 2 * kQueues * sizeof(int)) != hipSuccess ||
       hipMalloc(&c->d_flags, sizeof(uint32_t)) != hipSuccess) {
-    delete c;
+    acum_ctx_destroy(c);
     return ACUM_ERR_DEVICE;
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
-    delete c;
+    acum_ctx_destroy(c);
     return ACUM_ERR_DEVICE;
   }
   c->cu_count = prop.multiProcessorCount;
