@@ -904,7 +904,8 @@ int acum_run_write_bb(const acum_run* r, const char* dir, const float* BBoaspl, 
   if (!f) return fail(ACUM_ERR_IO, "cannot write bb_oaspl.out");
   std::fprintf(f, "nobs: %d\n", (int)r->nObs);
   for (int64_t i = 0; i < r->nObs; i++) std::fprintf(f, " %14.8E\n", (double)BBoaspl[i]);
-  std::fclose(f);
+  bool bad = std::ferror(f) != 0;  // read before the close
+  if ((std::fclose(f) != 0) || bad) return fail(ACUM_ERR_IO, "write failed: bb_oaspl.out");
   f = std::fopen(join(d, "bb_spl.dat").c_str(), "w");
   if (!f) return fail(ACUM_ERR_IO, "cannot write bb_spl.dat");
   const int nf2 = r->nf2;
@@ -928,7 +929,8 @@ int acum_run_write_bb(const acum_run* r, const char* dir, const float* BBoaspl, 
     }
     for (const std::string& s : rows) std::fwrite(s.data(), 1, s.size(), f);
   }
-  std::fclose(f);
+  bad = std::ferror(f) != 0;
+  if ((std::fclose(f) != 0) || bad) return fail(ACUM_ERR_IO, "write failed: bb_spl.dat");
   return ACUM_OK;
 }
 
