@@ -52,6 +52,21 @@ struct StagedCase {
   float* d_out[3] = {nullptr, nullptr, nullptr};
 };
 
+// Decay rate k of the exponential size profile of a case's tile groups (build_plan) for n items
+// per workgroup slot: the first (largest) group, k / (1 - e^-k) times the mean, is held to 0.8 n
+// mean groups, about one slot's share of the case; equal sizes (k = 0) up to 1.25 items per slot.
+double group_decay(double n) {
+  const double r = 0.8 * n;
+  if (r <= 1.0) return 0.0;
+  double lo = 1e-6, hi = 40.0;
+  for (int i = 0; i < 60; i++) {
+    const double m = 0.5 * (lo + hi);
+    if (m / -std::expm1(-m) < r) lo = m;
+    else hi = m;
+  }
+  return 0.5 * (lo + hi);
+}
+
 struct Plan {
   bool valid = false;
   acum_tune tune{};
@@ -491,7 +506,7 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     p.G[c] = G;
     p.slab_off[c] = p.slab_bytes;
     p.slab_bytes += ((G * row + 255) / 256) * 256;
-    // split the tiles into G runs of balanced element count
+    // split the tiles into G runs of decreasing element count
     int64_t tot = 0;
     for (auto& tl : ct[c]) tot += tl.ne;
     const int base = (int)p.tiles.size();
@@ -499,8 +514,15 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     const int nt = (int)ct[c].size();
     int t0 = 0;
     int64_t acc = 0;
+    // Group sizes fall off exponentially with the group index (density e^(-kx) over x in
+    // [0, 1]; at four items per slot k = 3 and the first group is about 20 times the last).
+    // The queues run large groups first, so the launch ends on small items and the workgroups
+    // finish close together.  k depends on this case and the device alone.
+    const double k = group_decay((double)G * nog / capacity);
+    auto cum = [k](double x) { return k > 0.0 ? std::expm1(-k * x) / std::expm1(-k) : x; };
     for (int g = 0; g < G; g++) {
-      const int64_t goal = tot * (g + 1) / G;
+      const double x = (double)(g + 1) / G;
+      const int64_t goal = (g == G - 1) ? tot : (int64_t)std::llround((double)tot * cum(x));
       int t1 = t0;
       int64_t grp = 0;
       // at least one tile per group; stop before overshooting the goal; leave one tile for
