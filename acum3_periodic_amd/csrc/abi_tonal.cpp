@@ -501,7 +501,10 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     int G = p.tune.groups > 0 ? p.tune.groups : (int)std::lround(target_items / nog);
     G = std::max(1, std::min<int>(G, (int)ct[c].size()));
     const int64_t row = sc.nObs * sc.nT * 2 * (int64_t)asz;
-    const int64_t cap = (int64_t)8 << 30;  // keep each case's partials under 8 GiB
+    // partial rows take at most a quarter of the case's element records, or 8 GiB if that is
+    // more: the bound scales with the case, so several cases of a batch still fit together
+    const int64_t rec_bytes = sc.nE * sc.nT * (int64_t)(sizeof(PosRec) + (mixed ? sizeof(AttrT<float>) : sizeof(AttrT<double>)));
+    const int64_t cap = std::max<int64_t>((int64_t)8 << 30, rec_bytes / 4);
     while (G > 1 && G * row > cap) G--;
     p.G[c] = G;
     p.slab_off[c] = p.slab_bytes;
