@@ -114,7 +114,9 @@ typedef struct acum_tune {
   int32_t window;      /* observer-time bins per lane in LDS: 16, 20, 24, 32 (+ 48, 64 mixed) */
   int32_t max_spread;  /* largest spread (bins) of a tile's elements around its first one     */
   int32_t max_tile;    /* most elements per tile                                              */
-  int32_t groups;      /* partial slabs per observer row (0 = automatic)                      */
+  int32_t groups;      /* partial slabs per observer row (0 = automatic: four work items per  */
+                       /* workgroup slot; either way at most what fits a quarter of the       */
+                       /* case's element records or 8 GiB, whichever is larger)               */
   int32_t max_chunk;   /* most source time steps per window pass                              */
   int32_t waves;       /* waves per workgroup: always 1 (one 64-observer wave per workgroup;  */
                        /* the LDS window, not the workgroup size, sets the occupancy)         */
