@@ -33,9 +33,9 @@ struct BpmPrepArgs {
 hipError_t launch_bpm_prep(hipStream_t s, const BpmPrepArgs& a);
 hipError_t launch_bpm_shape(hipStream_t s, const BpmShapeIn* in, int64_t nsrc, const float* ff, int nf2,
                             double a0, float* S, BpmSrcDev* dev);
-hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, int64_t nsrc, int64_t chunk,
-                          int nchunk, int nf2, const float* obs, int nObs, float vx, float vy, float vz,
-                          float aMv, float* part);
+hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, const int64_t* bounds, int nchunk,
+                          int nf2, const float* obs, int nObs, float vx, float vy, float vz, float aMv,
+                          float* part);
 hipError_t launch_bpm_final(hipStream_t s, const float* part, int nchunk, int nObs, int nf2, int nT, float* spl,
                             float* oaspl);
 int bpm_occupancy_blocks(int nf2);
@@ -142,15 +142,24 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   HIPCHK(hipGetDeviceProperties(&prop, ctx_device(ctx)));
   const int64_t target = 4LL * bpc * prop.multiProcessorCount;
   int64_t nchunk = std::max<int64_t>(1, target / std::max(1, obs_tiles * passes));
-  nchunk = std::min<int64_t>(nchunk, nsrc);
-  const int64_t chunk = (nsrc + nchunk - 1) / nchunk;
-  nchunk = (nsrc + chunk - 1) / chunk;
+  nchunk = std::min<int64_t>(std::min<int64_t>(nchunk, nsrc), 65535);
+  // source chunks of decreasing size (largest first), each at least one source
+  std::vector<int64_t> bounds(nchunk + 1, 0);
+  {
+    const double k = size_decay((double)obs_tiles * passes * nchunk / ((double)bpc * prop.multiProcessorCount));
+    for (int64_t i = 1; i < nchunk; i++) {
+      const int64_t b = (int64_t)std::llround((double)nsrc * size_profile(k, (double)i / nchunk));
+      bounds[i] = std::min(std::max(b, bounds[i - 1] + 1), nsrc - (nchunk - i));
+    }
+    bounds[nchunk] = nsrc;
+  }
   BpmSurfDev* d_surf;
   BpmShapeIn* d_in;
   BpmSrcDev* d_src;
   uint32_t* d_err;
   float *d_x, *d_bd, *d_st, *d_mt, *d_rt, *d_at, *d_tt;
   float *d_S, *d_ff, *d_obs, *d_part, *d_spl, *d_oa;
+  int64_t* d_bounds;
   // the observer kernel reads a whole pass of bands per source, past nf2 on the last source
   const size_t s_pad = (size_t)bpm_bands_per_pass(nf2) * 6;
   WorkspaceLayout L;
@@ -169,6 +178,7 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   L.add(&d_ff, nf2);
   L.add(&d_obs, (size_t)nObs * 3);
   L.add(&d_part, (size_t)nchunk * nObs * nf2 * 6);
+  L.add(&d_bounds, (size_t)nchunk + 1);
   L.add(&d_spl, (size_t)nObs * nf2 * 7);
   L.add(&d_oa, nObs);
   void* ws = nullptr;
@@ -186,6 +196,7 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   HIPCHK(hipMemcpyAsync(d_tt, c->bbdata_t, ntab * sizeof(float), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_ff, c->ff, nf2 * sizeof(float), hipMemcpyHostToDevice, s));
   HIPCHK(hipMemcpyAsync(d_obs, c->rObs, (size_t)nObs * 3 * sizeof(float), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d_bounds, bounds.data(), (nchunk + 1) * sizeof(int64_t), hipMemcpyHostToDevice, s));
   const double vx = c->vInf[0], vy = c->vInf[1], vz = c->vInf[2];
   const double aMv = c->a0 * c->a0 - vx * vx - vy * vy - vz * vz;
   HIPCHK(hipEventRecord(ctx_event(ctx, 0), s));
@@ -213,8 +224,8 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   HIPCHK(launch_bpm_prep(s, pa));
   HIPCHK(launch_bpm_shape(s, d_in, nsrc, d_ff, nf2, c->a0, d_S, d_src));
   HIPCHK(hipEventRecord(ctx_event(ctx, 1), s));
-  HIPCHK(launch_bpm_obs(s, d_src, d_S, nsrc, chunk, (int)nchunk, nf2, d_obs, nObs, (float)vx, (float)vy,
-                        (float)vz, (float)aMv, d_part));
+  HIPCHK(launch_bpm_obs(s, d_src, d_S, d_bounds, (int)nchunk, nf2, d_obs, nObs, (float)vx, (float)vy, (float)vz,
+                        (float)aMv, d_part));
   HIPCHK(hipEventRecord(ctx_event(ctx, 2), s));
   HIPCHK(launch_bpm_final(s, d_part, (int)nchunk, nObs, nf2, c->nTime, d_spl, d_oa));
   HIPCHK(hipEventRecord(ctx_event(ctx, 3), s));
@@ -241,6 +252,6 @@ extern "C" int acum_bpm(acum_ctx* ctx, const acum_bpm_case* c, float* BBspl, flo
   t->partial_bytes = (int64_t)nchunk * nObs * nf2 * 6 * 4;
   t->n_partials = (int32_t)nchunk;
   t->n_items = (int32_t)(obs_tiles * nchunk * passes);
-  t->launches = 2 + passes;
+  t->launches = 4;
   return ACUM_OK;
 }

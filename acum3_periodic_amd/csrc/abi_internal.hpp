@@ -1,6 +1,7 @@
 // Internal interface between the C-ABI translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <string>
 #include "../../include/acum.h"
 #include "dev_types.hpp"
@@ -11,6 +12,24 @@ hipStream_t ctx_stream(acum_ctx* c);
 hipEvent_t ctx_event(acum_ctx* c, int i);
 acum_timing* ctx_timing(acum_ctx* c);
 int ctx_device(acum_ctx* c);
+// Decay rate k of an exponential size profile (density e^(-kx) over x in [0, 1]) for work split
+// into pieces that run about n to a workgroup slot, largest first: the first piece, k / (1 - e^-k)
+// times the mean, is held to 0.8 n mean pieces, about one slot's share; equal sizes (k = 0) up to
+// 1.25 pieces per slot.  Cutting the work this way lets a launch end on small pieces.
+inline double size_decay(double n) {
+  const double r = 0.8 * n;
+  if (r <= 1.0) return 0.0;
+  double lo = 1e-6, hi = 40.0;
+  for (int i = 0; i < 60; i++) {
+    const double m = 0.5 * (lo + hi);
+    if (m / -std::expm1(-m) < r) lo = m;
+    else hi = m;
+  }
+  return 0.5 * (lo + hi);
+}
+// Fraction of the work in the first x of the pieces under that profile.
+inline double size_profile(double k, double x) { return k > 0.0 ? std::expm1(-k * x) / std::expm1(-k) : x; }
+
 // A device workspace owned by the context for the buffers of one call (broadband, spectra):
 // grown on demand, kept until the context is destroyed.  Calls on one context are serialised
 // and each synchronises its stream before returning, so one call's buffers never outlive it.

@@ -52,21 +52,6 @@ struct StagedCase {
   float* d_out[3] = {nullptr, nullptr, nullptr};
 };
 
-// Decay rate k of the exponential size profile of a case's tile groups (build_plan) for n items
-// per workgroup slot: the first (largest) group, k / (1 - e^-k) times the mean, is held to 0.8 n
-// mean groups, about one slot's share of the case; equal sizes (k = 0) up to 1.25 items per slot.
-double group_decay(double n) {
-  const double r = 0.8 * n;
-  if (r <= 1.0) return 0.0;
-  double lo = 1e-6, hi = 40.0;
-  for (int i = 0; i < 60; i++) {
-    const double m = 0.5 * (lo + hi);
-    if (m / -std::expm1(-m) < r) lo = m;
-    else hi = m;
-  }
-  return 0.5 * (lo + hi);
-}
-
 struct Plan {
   bool valid = false;
   acum_tune tune{};
@@ -521,8 +506,8 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     // [0, 1]; at four items per slot k = 3 and the first group is about 20 times the last).
     // The queues run large groups first, so the launch ends on small items and the workgroups
     // finish close together.  k depends on this case and the device alone.
-    const double k = group_decay((double)G * nog / capacity);
-    auto cum = [k](double x) { return k > 0.0 ? std::expm1(-k * x) / std::expm1(-k) : x; };
+    const double k = size_decay((double)G * nog / capacity);
+    auto cum = [k](double x) { return size_profile(k, x); };
     for (int g = 0; g < G; g++) {
       const double x = (double)(g + 1) / G;
       const int64_t goal = (g == G - 1) ? tot : (int64_t)std::llround((double)tot * cum(x));

@@ -224,11 +224,12 @@ __device__ __forceinline__ float max_nonneg(float a, float b) {
 // One lane per observer, a chunk of sources per workgroup, NB bands from band b0.
 template <int NB>
 __global__ __launch_bounds__(64) void bpm_obs_kernel(const BpmSrcDev* __restrict__ src, const float* __restrict__ S,
-                                                     int64_t nsrc, int64_t chunk, int nf2, int b0,
+                                                     const int64_t* __restrict__ bounds, int nf2,
                                                      const float* __restrict__ obs, int nObs, float vx, float vy,
                                                      float vz, float aMv, float inv_aMv, float* __restrict__ part) {
   const int o = blockIdx.x * 64 + threadIdx.x;
-  const int ch = blockIdx.y;
+  const int b0 = blockIdx.y * NB;  // first band of this pass
+  const int ch = blockIdx.z;       // source chunk [bounds[ch], bounds[ch + 1])
   const int oc = o < nObs ? o : nObs - 1;
   const float ox = obs[3 * oc], oy = obs[3 * oc + 1], oz = obs[3 * oc + 2];
   const int nb = min(NB, nf2 - b0);
@@ -237,7 +238,7 @@ __global__ __launch_bounds__(64) void bpm_obs_kernel(const BpmSrcDev* __restrict
   for (int b = 0; b < NB; b++)
 #pragma unroll
     for (int c = 0; c < kComp; c++) acc[b][c] = 0.0f;
-  const int64_t s0 = ch * chunk, s1 = min(nsrc, s0 + chunk);
+  const int64_t s0 = bounds[ch], s1 = bounds[ch + 1];
   const float floor_e = 1.0e-10f, floor_t = 2.2204460492503131e-16f;
   for (int64_t s = s0; s < s1; s++) {
     const BpmSrcDev q = src[s];
@@ -373,26 +374,24 @@ int bpm_bands_per_pass(int nf2) {
   return nb <= 12 ? 12 : (nb <= 14 ? 14 : 17);
 }
 
-hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, int64_t nsrc, int64_t chunk,
-                          int nchunk, int nf2, const float* obs, int nObs, float vx, float vy, float vz,
-                          float aMv, float* part) {
+// One launch over (observer tile, band pass, source chunk), the chunk slowest: blocks dispatch in
+// index order, so with chunks of decreasing size the launch ends on the smallest ones.
+hipError_t launch_bpm_obs(hipStream_t s, const BpmSrcDev* src, const float* S, const int64_t* bounds, int nchunk,
+                          int nf2, const float* obs, int nObs, float vx, float vy, float vz, float aMv,
+                          float* part) {
   const float inv_aMv = (float)(1.0 / (double)aMv);
   const int NB = bpm_bands_per_pass(nf2);
-  for (int b0 = 0; b0 < nf2; b0 += NB) {
-    dim3 g((nObs + 63) / 64, nchunk);
-    if (NB == 17)
-      hipLaunchKernelGGL(bpm_obs_kernel<17>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
-                         vz, aMv, inv_aMv, part);
-    else if (NB == 14)
-      hipLaunchKernelGGL(bpm_obs_kernel<14>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
-                         vz, aMv, inv_aMv, part);
-    else
-      hipLaunchKernelGGL(bpm_obs_kernel<12>, g, dim3(64), 0, s, src, S, nsrc, chunk, nf2, b0, obs, nObs, vx, vy,
-                         vz, aMv, inv_aMv, part);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  const dim3 g((nObs + 63) / 64, (nf2 + NB - 1) / NB, nchunk);
+  if (NB == 17)
+    hipLaunchKernelGGL(bpm_obs_kernel<17>, g, dim3(64), 0, s, src, S, bounds, nf2, obs, nObs, vx, vy, vz, aMv,
+                       inv_aMv, part);
+  else if (NB == 14)
+    hipLaunchKernelGGL(bpm_obs_kernel<14>, g, dim3(64), 0, s, src, S, bounds, nf2, obs, nObs, vx, vy, vz, aMv,
+                       inv_aMv, part);
+  else
+    hipLaunchKernelGGL(bpm_obs_kernel<12>, g, dim3(64), 0, s, src, S, bounds, nf2, obs, nObs, vx, vy, vz, aMv,
+                       inv_aMv, part);
+  return hipGetLastError();
 }
 
 hipError_t launch_bpm_final(hipStream_t s, const float* part, int nchunk, int nObs, int nf2, int nT, float* spl,

@@ -188,7 +188,7 @@ def summarize_stats(d, workload="rotor"):
                 "# 34 bands (one warm-up and three timed calls)"]
         cmd = ["    python3 $R/bench.py --workload bpm --steps 3",
                "python3 $R/bench.py --workload bpm --summarize-stats $OUT/prof"]
-        note_ = ["`bpm_obs_kernel<17>` runs twice per call (34 bands in two passes of 17).", ""]
+        note_ = ["`bpm_obs_kernel<17>` covers the 34 bands in two passes of 17 within one launch per call.", ""]
         out = os.path.join(ROOT, "profiles", "bpm_c3_kernel_stats.md")
     else:
         head = ["# Kernel statistics: tonal path, four-blade rotor (4 x 20,000 elements, 1024 steps,",
