@@ -521,11 +521,20 @@ def main():
     oT, oL, oA = O.tonal_f64(sub, threads)
     cpu_s = time.perf_counter() - t
     cpu_evals = sum(E) * case.nTime * nob
+    # the float instance of the same restatement (the reference's float arithmetic), same sample
+    note(f"CPU float oracle on {nob} observers")
+    sub.precision = acum.MIXED
+    t = time.perf_counter()
+    fT, fL, fA = O.tonal_f32(sub, threads)
+    cpu32_s = time.perf_counter() - t
 
     gA = mixed[2][:, :nob]
     d_oaspl = np.abs(acum.oaspl(gA) - acum.oaspl(oA.astype(np.float32)))
     acc = dict(observers=nob, mixed_vs_oracle_rel_l2_pA=O.rel_l2(gA, oA),
-               mixed_vs_oracle_max_doaspl_db=float(d_oaspl.max()))
+               mixed_vs_oracle_max_doaspl_db=float(d_oaspl.max()),
+               float_oracle_vs_oracle_rel_l2_pA=O.rel_l2(fA, oA),
+               float_oracle_vs_oracle_max_doaspl_db=float(
+                   np.abs(acum.oaspl(fA) - acum.oaspl(oA.astype(np.float32))).max()))
     if fp64:
         acc["fp64_vs_oracle_rel_l2_pA"] = O.rel_l2(plain[2][:, :nob], oA)
         d = np.abs(acum.oaspl(mixed[2]) - acum.oaspl(plain[2]))
@@ -576,7 +585,8 @@ def main():
         "cpu_baseline": {"value": round(cpu_evals / cpu_s / 1e9, 4), "unit": "G src-obs-time evals/s",
                          "cores": threads, "kind": "port",
                          "sample": f"FP64 oracle, same workload, first {nob} observers "
-                                   f"({cpu_evals:.3g} evals, {cpu_s:.1f} s)"},
+                                   f"({cpu_evals:.3g} evals, {cpu_s:.1f} s)",
+                         "float_instance_value": round(cpu_evals / cpu32_s / 1e9, 4)},
         "binning": {"kernel": "tonal_reduce_kernel<float>", "ms": round(red_ms, 4),
                     "bytes": bin_bytes, "GBps": round(bin_bytes / (red_ms * 1e-3) / 1e9, 1),
                     "frac_hbm": round(bin_bytes / (red_ms * 1e-3) / HBM_PEAK, 4),
