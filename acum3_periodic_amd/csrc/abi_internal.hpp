@@ -34,6 +34,10 @@ inline double size_profile(double k, double x) { return k > 0.0 ? std::expm1(-k 
 // grown on demand, kept until the context is destroyed.  Calls on one context are serialised
 // and each synchronises its stream before returning, so one call's buffers never outlive it.
 hipError_t ctx_workspace(acum_ctx* c, size_t bytes, void** out);
+// Host <-> device copies of large pageable buffers through the context's pinned staging buffers
+// (abi_tonal.cpp); the upload is queued on the context's stream, the download waits for its data.
+hipError_t ctx_upload(acum_ctx* c, void* dst, const void* src, size_t n);
+hipError_t ctx_download(acum_ctx* c, void* dst, const void* src, size_t n);
 
 // Lays out a call's device buffers (at most 32) in one workspace, each at a 256-byte boundary.
 struct WorkspaceLayout {

@@ -53,7 +53,7 @@ extern "C" int acum_preprocess_gpu(acum_ctx* ctx, acum_run* r) {
   for (int i = 0; i < 4; i++) HIPCHK(hipMalloc(&d.p[i], (size_t)q * width[i] * sizeof(float)));
   HIPCHK(hipEventRecord(ctx_event(ctx, 0), st));
   for (int i : {0, 1, 3})
-    HIPCHK(hipMemcpyAsync(d.p[i], host[i], (size_t)q * width[i] * sizeof(float), hipMemcpyHostToDevice, st));
+    HIPCHK(ctx_upload(ctx, d.p[i], host[i], (size_t)q * width[i] * sizeof(float)));
   PrepArgs a{};
   a.X = d.p[0];
   a.N = d.p[1];
@@ -72,7 +72,7 @@ extern "C" int acum_preprocess_gpu(acum_ctx* ctx, acum_run* r) {
   HIPCHK(launch_preprocess(st, a, grid));
   HIPCHK(hipEventRecord(ctx_event(ctx, 2), st));
   for (int i = 0; i < 4; i++)
-    HIPCHK(hipMemcpyAsync(host[i], d.p[i], (size_t)q * width[i] * sizeof(float), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx_download(ctx, host[i], d.p[i], (size_t)q * width[i] * sizeof(float)));
   HIPCHK(hipEventRecord(ctx_event(ctx, 3), st));
   HIPCHK(hipStreamSynchronize(st));
   float up = 0, kern = 0, down = 0;

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <omp.h>
 #include <string>
 #include <vector>
 #include "abi_internal.hpp"
@@ -95,6 +96,8 @@ struct acum_ctx {
   int next_handle = 1;
   void* ws = nullptr;  // per-call workspace (ctx_workspace)
   size_t ws_bytes = 0;
+  void* pin[2] = {nullptr, nullptr};  // pinned staging buffers (ctx_upload / ctx_download)
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
 };
 
 namespace acum {
@@ -108,6 +111,94 @@ hipStream_t ctx_stream(acum_ctx* c) { return c->stream; }
 hipEvent_t ctx_event(acum_ctx* c, int i) { return c->ev[i]; }
 acum_timing* ctx_timing(acum_ctx* c) { return &c->timing; }
 int ctx_device(acum_ctx* c) { return c->device; }
+
+namespace {
+
+constexpr size_t kStageChunk = (size_t)64 << 20;  // bytes per pinned staging buffer
+constexpr size_t kStageMin = (size_t)8 << 20;     // smaller copies go straight from pageable memory
+
+// host memcpy split over up to 16 threads
+void par_memcpy(void* dst, const void* src, size_t n) {
+  const size_t piece = (size_t)1 << 20;
+  const int64_t np = (int64_t)((n + piece - 1) / piece);
+  const int nt = (int)std::min<int64_t>(np, std::min(16, omp_get_max_threads()));
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (int64_t i = 0; i < np; i++) {
+    const size_t o = (size_t)i * piece;
+    std::memcpy((char*)dst + o, (const char*)src + o, std::min(piece, n - o));
+  }
+}
+
+hipError_t ensure_pinned(acum_ctx* c) {
+  for (int b = 0; b < 2; b++) {
+    if (!c->pin[b]) {
+      const hipError_t e = hipHostMalloc(&c->pin[b], kStageChunk, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        c->pin[b] = nullptr;
+        return e;
+      }
+    }
+    if (!c->pin_ev[b]) {
+      hipError_t e = hipEventCreateWithFlags(&c->pin_ev[b], hipEventDisableTiming);
+      if (e != hipSuccess) {
+        c->pin_ev[b] = nullptr;
+        return e;
+      }
+      if ((e = hipEventRecord(c->pin_ev[b], c->stream)) != hipSuccess) return e;  // a completed event
+    }
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+// Large host-to-device copies from pageable memory go through the context's two pinned
+// buffers: a parallel host copy into one buffer overlaps the DMA out of the other.  The source
+// may be reused when the call returns; the DMAs complete in stream order.
+hipError_t ctx_upload(acum_ctx* c, void* dst, const void* src, size_t n) {
+  if (n < kStageMin) return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, c->stream);
+  hipError_t e = ensure_pinned(c);
+  if (e != hipSuccess) return e;
+  for (size_t off = 0, i = 0; off < n; off += kStageChunk, i++) {
+    const int b = (int)(i & 1);
+    const size_t len = std::min(kStageChunk, n - off);
+    if ((e = hipEventSynchronize(c->pin_ev[b])) != hipSuccess) return e;  // its last DMA is done
+    par_memcpy(c->pin[b], (const char*)src + off, len);
+    if ((e = hipMemcpyAsync((char*)dst + off, c->pin[b], len, hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+      return e;
+    if ((e = hipEventRecord(c->pin_ev[b], c->stream)) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Large device-to-host copies into pageable memory, the reverse way; returns when dst holds the
+// data (the copy follows the work queued before it on the context's stream).
+hipError_t ctx_download(acum_ctx* c, void* dst, const void* src, size_t n) {
+  hipError_t e;
+  if (n < kStageMin) {
+    if ((e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, c->stream)) != hipSuccess) return e;
+    return hipStreamSynchronize(c->stream);
+  }
+  if ((e = ensure_pinned(c)) != hipSuccess) return e;
+  size_t prev_off = 0, prev_len = 0;
+  for (size_t off = 0, i = 0; off < n + kStageChunk; off += kStageChunk, i++) {
+    const int b = (int)(i & 1);
+    if (off < n) {  // queue this chunk's DMA into buffer b (emptied by the host before)
+      const size_t len = std::min(kStageChunk, n - off);
+      if ((e = hipMemcpyAsync(c->pin[b], (const char*)src + off, len, hipMemcpyDeviceToHost, c->stream)) !=
+          hipSuccess)
+        return e;
+      if ((e = hipEventRecord(c->pin_ev[b], c->stream)) != hipSuccess) return e;
+    }
+    if (prev_len) {  // empty the other buffer, filled by the previous chunk's DMA
+      if ((e = hipEventSynchronize(c->pin_ev[1 - b])) != hipSuccess) return e;
+      par_memcpy((char*)dst + prev_off, c->pin[1 - b], prev_len);
+    }
+    prev_off = off;
+    prev_len = off < n ? std::min(kStageChunk, n - off) : 0;
+  }
+  return hipSuccess;
+}
 
 hipError_t ctx_workspace(acum_ctx* c, size_t bytes, void** out) {
   if (bytes > c->ws_bytes) {
@@ -235,8 +326,7 @@ int stage_case(acum_ctx* ctx, const acum_tonal_case* c, StagedCase& sc) {
     int64_t e0 = 0;
     for (int64_t s = 0; s < c->nSurf; s++) {
       const size_t n = (size_t)c->E[s] * nT * width[a];
-      HIPCHK(ctx, hipMemcpyAsync(sc.d_aos[a] + e0 * nT * width[a], src[a] + off[a][s],
-                                 n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+      HIPCHK(ctx, ctx_upload(ctx, sc.d_aos[a] + e0 * nT * width[a], src[a] + off[a][s], n * sizeof(float)));
       e0 += c->E[s];
     }
   }
@@ -311,8 +401,7 @@ int stage_raw_run(acum_ctx* ctx, const acum_run* r, StagedCase& sc) {
   for (int a = 0; a < 4; a++) {
     HIPCHK(ctx, hipMalloc(&sc.d_aos[a], (size_t)q * width[a] * sizeof(float)));
     if (a != 2)  // V is written by the preprocessing
-      HIPCHK(ctx, hipMemcpyAsync(sc.d_aos[a], src[a], (size_t)q * width[a] * sizeof(float),
-                                 hipMemcpyHostToDevice, ctx->stream));
+      HIPCHK(ctx, ctx_upload(ctx, sc.d_aos[a], src[a], (size_t)q * width[a] * sizeof(float)));
   }
   PrepArgs pa{};
   pa.X = sc.d_aos[0];
@@ -811,6 +900,11 @@ void acum_ctx_destroy(acum_ctx* c) {
   (void)hipSetDevice(c->device);
   for (auto& kv : c->stages) free_stage(*kv.second);
   (void)hipFree(c->ws);
+  for (int b = 0; b < 2; b++) {
+    if (c->pin_ev[b]) (void)hipEventSynchronize(c->pin_ev[b]);
+    if (c->pin[b]) (void)hipHostFree(c->pin[b]);
+    if (c->pin_ev[b]) (void)hipEventDestroy(c->pin_ev[b]);
+  }
   (void)hipFree(c->d_counter);
   (void)hipFree(c->d_flags);
   for (auto& e : c->ev)
@@ -876,7 +970,7 @@ int acum_tonal_fetch(acum_ctx* ctx, int32_t handle, int32_t icase, float* pT, fl
   const size_t n = (size_t)sc.nT * sc.nObs * sizeof(float);
   float* dst[3] = {pT, pL, pA};
   for (int i = 0; i < 3; i++)
-    if (dst[i]) HIPCHK(ctx, hipMemcpy(dst[i], sc.d_out[i], n, hipMemcpyDeviceToHost));
+    if (dst[i]) HIPCHK(ctx, ctx_download(ctx, dst[i], sc.d_out[i], n));
   return ACUM_OK;
 }
 
