@@ -38,6 +38,20 @@ def test_gpu_preprocess_bit_identical(ctx, config, over):
         assert np.array_equal(ca[k].view(np.uint32), cb[k].view(np.uint32)), k
 
 
+def test_gpu_preprocess_staged_in_chunks(ctx):
+    """Arrays of 97-145 MB, neither a multiple of the 64 MB pinned staging buffers nor of the
+    1 MB host-copy pieces, go up and come back through several staging rounds; the results are
+    bit-identical to the host preprocessing."""
+    a, b = _pair(2, nChord=40, nSpan=37, nTime=512)
+    a.preprocess()
+    b.preprocess_gpu(ctx)
+    ca = acum.case_arrays(a.tonal_case(acum.FP64))
+    assert ca["X"].nbytes > (64 << 20) and ca["X"].nbytes % (1 << 20) != 0
+    cb = acum.case_arrays(b.tonal_case(acum.FP64))
+    for k in ("X", "N", "V", "PR"):
+        assert np.array_equal(ca[k].view(np.uint32), cb[k].view(np.uint32)), k
+
+
 @pytest.mark.parametrize("lowpass,periodic", [(1, 1), (0, 0), (1, 0)])
 def test_gpu_preprocess_filters_and_ends(ctx, tmp_path, lowpass, periodic):
     d = tmp_path / "run"
