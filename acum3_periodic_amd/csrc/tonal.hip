@@ -228,6 +228,8 @@ __device__ __forceinline__ int sign_word(double x) { return (int)(__double_as_lo
 // zero: a weight is positive exactly when its bin lies inside the tent, so the clamp selects the
 // tent's bins (up to the rounding of the weight itself, below 1e-6 in the mixed path).  Sides
 // longer than two bins go to a wave-uniform tail that counts their bins from the FP64 times.
+constexpr int kFlush = 10;  // window slots per round of the slab update
+
 template <bool MIXED, bool MOVING, int W>
 __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
                                                         const Tile* __restrict__ tiles,
@@ -453,16 +455,46 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
         // add the window into this item's slab row and clear it (untouched slots hold zeros)
         int b = base % nT;
         if (b < 0) b += nT;
-        for (int s = 0; s < W; s++) {
-          const R2 v = win[s * 64 + lane];
-          if (v.x != (R)0 || v.y != (R)0) {
-            win[s * 64 + lane] = R2{(R)0, (R)0};
-            if (active) {
-              slab[2 * b] += v.x;
-              slab[2 * b + 1] += v.y;
+        if (!MIXED && W <= nT) {
+          // FP64 path (two waves per SIMD, 16-byte slots): the W bins are distinct, so each round
+          // loads the touched slots' slab values before adding to any and waits for one memory
+          // latency instead of one per slot (8% faster there; the mixed path, with four waves per
+          // SIMD to hide the latency, ran 1.3% slower this way and keeps the slot-by-slot loop)
+          R2* row = (R2*)slab;
+          for (int s0 = 0; s0 < W; s0 += kFlush) {
+            R2 v[kFlush], old[kFlush];
+            int bs[kFlush];
+            bool nz[kFlush];
+#pragma unroll
+            for (int j = 0; j < kFlush; j++) {
+              const int s = s0 + j;
+              nz[j] = false;
+              if (s < W) {
+                v[j] = win[s * 64 + lane];
+                nz[j] = active && (v[j].x != (R)0 || v[j].y != (R)0);  // untouched slots stay out
+                win[s * 64 + lane] = R2{(R)0, (R)0};
+                bs[j] = (b + s >= nT) ? b + s - nT : b + s;
+              }
             }
+#pragma unroll
+            for (int j = 0; j < kFlush; j++)
+              if (nz[j]) old[j] = row[bs[j]];
+#pragma unroll
+            for (int j = 0; j < kFlush; j++)
+              if (nz[j]) row[bs[j]] = R2{old[j].x + v[j].x, old[j].y + v[j].y};
           }
-          b = (b + 1 == nT) ? 0 : b + 1;
+        } else {  // slot by slot (also where a period shorter than the window repeats bins)
+          for (int s = 0; s < W; s++) {
+            const R2 v = win[s * 64 + lane];
+            if (v.x != (R)0 || v.y != (R)0) {
+              win[s * 64 + lane] = R2{(R)0, (R)0};
+              if (active) {
+                slab[2 * b] += v.x;
+                slab[2 * b + 1] += v.y;
+              }
+            }
+            b = (b + 1 == nT) ? 0 : b + 1;
+          }
         }
       }
     }
