@@ -39,6 +39,7 @@ CASES = [
     ("hover_odd_obs", 1, dict(nObs=70, nTime=40)),
     ("flight", 2, {}),
     ("flight_coarse_time", 2, dict(nTime=24)),
+    ("hover_period_below_window", 1, dict(nTime=16)),  # 16 steps < the 20-bin window: bins repeat
 ]
 
 
