@@ -292,16 +292,25 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
         const int km0 = (k0 > 0) ? k0 - 1 : nT - 1;
         const int kn0 = (k0 + 1 < nT) ? k0 + 1 : 0;
         int base;
+        // the two positions an element's pass starts from (steps k0-1 and k0); the next
+        // element's are loaded (through vector registers) while this element's steps run
+        PosRec nm = load_words<PosRec>(pos + tl.e0 * nT + km0), nc = load_words<PosRec>(pos + tl.e0 * nT + k0);
         {
-          const PosRec p = load_words<PosRec>(pos + tl.e0 * nT + km0);
+          const PosRec& p = nm;
           double d;
           double tb = obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &d);
           if (k0 == 0) tb = tb - dnT;
           base = (int)floor(tb) - tl.spread - 1;
         }
+        const cptr<PosRec> vpos = (cptr<PosRec>)(uintptr_t)Cp->pos;
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
           const PosPtr pe = pos + e * nT;
           const AttrPtr ae = attr + e * nT;
+          const PosRec pm_e = nm, pc_e = nc;
+          if (e + 1 < tl.e0 + tl.ne) {
+            nm = load_words<PosRec>(vpos + (e + 1) * nT + km0);
+            nc = load_words<PosRec>(vpos + (e + 1) * nT + k0);
+          }
           // Per-step state, kept twice and used in turn (cur is the step being evaluated, prv the
           // one before it), as are the record buffers: the loop runs two steps per iteration with
           // the roles swapped, so nothing is copied from one step to the next.
@@ -314,10 +323,10 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
           R dxc, dyc, dzc, delc;  // obs - src and the delay of the step being evaluated
           {
             double dl;
-            const PosRec p = load_words<PosRec>(pe + km0);
+            const PosRec& p = pm_e;
             double tp = obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &dl);
             if (k0 == 0) tp = tp - dnT;  // the step before step 0 is step nT-1 of the previous period
-            const PosRec q = load_words<PosRec>(pe + k0);
+            const PosRec& q = pc_e;
             const double dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
             const double tc = obs_time<MOVING>(k, q.jt, dx, dy, dz, &dl);
             dxc = (R)dx;
