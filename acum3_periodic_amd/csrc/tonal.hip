@@ -357,6 +357,10 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
                           const Rec& ac, Rec& an) __attribute__((always_inline)) {
             const int kn = (kk + 1 < nT) ? kk + 1 : 0;
             const int knn = (kn + 1 < nT) ? kn + 1 : 0;
+            // Wait for the loads of the step before (issued a step ago, so complete) before issuing
+            // this step's: scalar loads return out of order, so a wait placed after the new loads
+            // would wait for them too.  lgkmcnt(0), vmcnt and expcnt unlimited.
+            __builtin_amdgcn_s_waitcnt(0xc07f);
             pnn = load_words<PosRec>(pe + knn);
             an = load_rec(kn);
             // the footprint is known before the solve: start its window reads now
