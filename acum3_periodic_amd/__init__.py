@@ -55,7 +55,8 @@ class Timing(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("window", C.c_int32), ("max_spread", C.c_int32), ("max_tile", C.c_int32),
-                ("groups", C.c_int32), ("max_chunk", C.c_int32), ("waves", C.c_int32)]
+                ("groups", C.c_int32), ("max_chunk", C.c_int32), ("waves", C.c_int32),
+                ("motion_permille", C.c_int32)]
 
 
 class Inputs(C.Structure):

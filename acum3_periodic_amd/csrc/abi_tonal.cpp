@@ -19,7 +19,7 @@ namespace acum {
 hipError_t launch_preprocess(hipStream_t s, const PrepArgs& a, int grid);
 hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
                              const TonalCaseDev* cases, const Tile* tiles, const Item* items,
-                             const int* qoff, int* counter, uint32_t* flags);
+                             const int* qoff, int* counter);
 hipError_t eval_occupancy(bool mixed, bool moving, int W, int* blocks_per_cu);
 bool eval_window_supported(bool mixed, int W);
 hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* cases, int icase,
@@ -51,7 +51,12 @@ struct StagedCase {
   PosRec* d_pos = nullptr;                                  // shared by both precisions
   void* d_attr[2] = {nullptr, nullptr};                     // [ACUM_FP64], [ACUM_MIXED]
   float* d_out[3] = {nullptr, nullptr, nullptr};
+  bool full_motion = false;  // plan with the full source-motion bound (set after a window overflow)
 };
+
+// Default of acum_tune.motion_permille: on the four-blade rotor a bound of 0.6 holds for every
+// tile and observer (0.45 overflows) and lengthens the passes from ~5 to ~6 steps.
+constexpr int kMotionPermille = 600;
 
 struct Plan {
   bool valid = false;
@@ -65,6 +70,7 @@ struct Plan {
   Item* d_items = nullptr;
   TonalCaseDev* d_cases = nullptr;
   void* d_slab = nullptr;
+  uint32_t* d_flags = nullptr;  // [cases] error flags of the last pass
   int n_still = 0;  // items[0, n_still) are still-air cases, the rest moving-medium ones
   int grid[2] = {0, 0};
   // queue q of launch l (0 still air, 1 moving) holds items [qoff[l][q], qoff[l][q+1]) of that
@@ -240,6 +246,7 @@ void free_plan(Plan& p) {
   (void)hipFree(p.d_qoff);
   (void)hipFree(p.d_cases);
   (void)hipFree(p.d_slab);
+  (void)hipFree(p.d_flags);
   p = Plan();
 }
 
@@ -476,7 +483,9 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
 // so K step_bins + 2 spread + 5 <= W and (K + 1) step_bins + 2 spread + 3 <= W.
 int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vector<Tile>& tiles) {
   const int64_t nT = sc.nT;
-  const double L = sc.k.r2b / (1.0 - sc.Minf);  // bins per metre
+  // bins of observer time per metre of source motion: the full bound, or the reduced one
+  const double motion = sc.full_motion ? 1.0 : (t.motion_permille > 0 ? t.motion_permille : kMotionPermille) / 1000.0;
+  const double L = sc.k.r2b / (1.0 - sc.Minf) * motion;
   const int W = t.window;
   const float* P = sc.pos.data();
   auto dist2 = [&](int64_t e1, int64_t k1, int64_t e2, int64_t k2) {
@@ -687,6 +696,7 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
   HIPCHK(ctx, hipMalloc(&p.d_qoff, sizeof(p.qoff)));
   HIPCHK(ctx, hipMalloc(&p.d_cases, nc * sizeof(TonalCaseDev)));
   HIPCHK(ctx, hipMalloc(&p.d_slab, p.slab_bytes));
+  HIPCHK(ctx, hipMalloc(&p.d_flags, nc * sizeof(uint32_t)));
   std::vector<TonalCaseDev> hc(nc);
   for (int c = 0; c < nc; c++) {
     const StagedCase& sc = st.cases[c];
@@ -701,6 +711,7 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     d.nE = sc.nE;
     d.nObs = (int32_t)sc.nObs;
     d.G = p.G[c];
+    d.flags = p.d_flags + c;
     d.k = sc.k;
   }
   HIPCHK(ctx, hipMemcpyAsync(p.d_tiles, p.tiles.data(), p.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
@@ -721,16 +732,17 @@ int enqueue_pass(acum_ctx* ctx, Stage& st, int prec, bool with_events) {
   hipStream_t s = ctx->stream;
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
   HIPCHK(ctx, launch_reset(s, ctx->d_counter, ctx->d_flags));
+  HIPCHK(ctx, launch_fill_zero(s, p.d_flags, (int64_t)st.cases.size() * sizeof(uint32_t)));
   HIPCHK(ctx, launch_fill_zero(s, p.d_slab, p.slab_bytes));
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
   const int nmov = (int)p.items.size() - p.n_still;
   if (p.n_still > 0)
     HIPCHK(ctx, launch_tonal_eval(mixed, false, p.tune.window, p.grid[0], s, p.d_cases, p.d_tiles,
-                                  p.d_items, p.d_qoff, ctx->d_counter, ctx->d_flags));
+                                  p.d_items, p.d_qoff, ctx->d_counter));
   if (nmov > 0)
     HIPCHK(ctx, launch_tonal_eval(mixed, true, p.tune.window, p.grid[1], s, p.d_cases, p.d_tiles,
                                   p.d_items + p.n_still, p.d_qoff + (kQueues + 1),
-                                  ctx->d_counter + kQueues, ctx->d_flags));
+                                  ctx->d_counter + kQueues));
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
   for (int c = 0; c < (int)st.cases.size(); c++)
     HIPCHK(ctx, launch_tonal_reduce(mixed, s, p.d_cases, c, (int)st.cases[c].nT, (int)st.cases[c].nObs));
@@ -738,14 +750,28 @@ int enqueue_pass(acum_ctx* ctx, Stage& st, int prec, bool with_events) {
   return ACUM_OK;
 }
 
-int check_flags(acum_ctx* ctx) {
-  uint32_t f = 0;
-  HIPCHK(ctx, hipMemcpy(&f, ctx->d_flags, sizeof(f), hipMemcpyDeviceToHost));
-  if (f & kFlagWindow)
-    return ctx_fail(ctx, ACUM_ERR_PLAN, "a contribution fell outside its tile window");
-  if (f & (kFlagNumeric | kFlagSweep))
+// Reads the cases' error flags after a pass.  A case planned with the reduced source-motion bound
+// whose window overflowed is marked for the full bound (*retry); any other error fails the run.
+int check_flags(acum_ctx* ctx, Stage& st, int prec, bool* retry) {
+  const int nc = (int)st.cases.size();
+  std::vector<uint32_t> f(nc, 0u);
+  HIPCHK(ctx, hipMemcpy(f.data(), st.plan[prec].d_flags, nc * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  *retry = false;
+  uint32_t all = 0;
+  for (int c = 0; c < nc; c++) {
+    if ((f[c] & kFlagWindow) && !st.cases[c].full_motion) {
+      st.cases[c].full_motion = true;
+      *retry = true;
+      f[c] &= ~kFlagWindow;
+    }
+    all |= f[c];
+  }
+  if (all & (kFlagNumeric | kFlagSweep))
     return ctx_fail(ctx, ACUM_ERR_NUMERIC,
                     "non-finite observer time, source on an observer, or |Mr| >= 1");
+  if (*retry) return ACUM_OK;
+  if (all & kFlagWindow)
+    return ctx_fail(ctx, ACUM_ERR_PLAN, "a contribution fell outside its tile window");
   return ACUM_OK;
 }
 
@@ -754,7 +780,7 @@ Stage* get_stage(acum_ctx* ctx, int32_t handle) {
   return it == ctx->stages.end() ? nullptr : it->second.get();
 }
 
-int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
+int run_stage_once(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph, bool* retry) {
   if (prec != ACUM_FP64 && prec != ACUM_MIXED) return ctx_fail(ctx, ACUM_ERR_ARG, "bad precision");
   if (repeats < 1) repeats = 1;
   struct Events {  // destroyed on every return
@@ -833,7 +859,22 @@ int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
   for (int g : p.G) t.n_partials = std::max(t.n_partials, g);
   t.n_items = (int32_t)p.items.size();
   t.launches = 3 + (int32_t)st.cases.size();
-  return check_flags(ctx);
+  return check_flags(ctx, st, prec, retry);
+}
+
+// Tiles are planned with a reduced bound on how far a source moves in observer time per step
+// (acum_tune.motion_permille), which lengthens the passes; the kernel checks every footprint
+// against its window, so a case whose bound was too tight is planned again with the full bound
+// and the pass is run again.  The plan of a case depends only on the case, so results agree
+// between a case run alone and in a batch.
+int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
+  bool retry = false;
+  int rc = run_stage_once(ctx, st, prec, repeats, use_graph, &retry);
+  if (rc || !retry) return rc;
+  for (auto& p : st.plan) free_plan(p);  // both precisions' plans hold the case's tiles
+  rc = run_stage_once(ctx, st, prec, repeats, use_graph, &retry);
+  if (!rc && retry) return ctx_fail(ctx, ACUM_ERR_PLAN, "a contribution fell outside its tile window");
+  return rc;
 }
 
 acum_tune default_tune(int prec) {
@@ -848,6 +889,7 @@ acum_tune default_tune(int prec) {
   t.groups = 0;
   t.max_chunk = 1 << 20;
   t.waves = 1;
+  t.motion_permille = kMotionPermille;
   return t;
 }
 
@@ -1069,6 +1111,8 @@ int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t) {
     return ctx_fail(ctx, ACUM_ERR_ARG, "window must be 16, 20, 24 or 32 (or 48, 64 for the mixed path)");
   if (t->max_spread < 1 || t->max_tile < 1 || t->max_chunk < 1 || t->groups < 0)
     return ctx_fail(ctx, ACUM_ERR_ARG, "bad tile limits");
+  if (t->motion_permille != 0 && (t->motion_permille < 100 || t->motion_permille > 1000))
+    return ctx_fail(ctx, ACUM_ERR_ARG, "motion_permille must be 0 (default) or 100..1000");
   ctx->tune[precision] = *t;
   ctx->tune[precision].waves = 1;
   return ACUM_OK;

@@ -54,6 +54,7 @@ struct TonalCaseDev {
   float* pA;
   int64_t nE;
   int32_t nObs, G;
+  uint32_t* flags;      // this case's error flags (kFlag*), cleared before each pass
   TimeConst k;
 };
 

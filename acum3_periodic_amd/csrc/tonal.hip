@@ -228,15 +228,23 @@ __device__ __forceinline__ int sign_word(double x) { return (int)(__double_as_lo
 // zero: a weight is positive exactly when its bin lies inside the tent, so the clamp selects the
 // tent's bins (up to the rounding of the weight itself, below 1e-6 in the mixed path).  Sides
 // longer than two bins go to a wave-uniform tail that counts their bins from the FP64 times.
-constexpr int kFlush = 10;  // window slots per round of the slab update
+constexpr int kFlush = 5;  // window slots per round of the FP64 path's slab update
+
+// Waves per SIMD that the LDS window allows (160 KB per CU, four SIMDs, one wave per workgroup):
+// four for the mixed path's float2 window at W = 20, two for the FP64 path's double2 one.  The
+// register budget is held to it.
+constexpr int window_waves(bool mixed, int W) {
+  const int per_cu = 160 * 1024 / (W * 64 * (mixed ? 8 : 16));
+  return per_cu / 4 > 4 ? 4 : (per_cu / 4 < 1 ? 1 : per_cu / 4);
+}
 
 template <bool MIXED, bool MOVING, int W>
-__global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(window_waves(MIXED, W))))
+void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
                                                         const Tile* __restrict__ tiles,
                                                         const Item* __restrict__ items,
                                                         const int* __restrict__ qoff,
-                                                        int* __restrict__ counter,
-                                                        uint32_t* __restrict__ flags_out) {
+                                                        int* __restrict__ counter) {
   using P = Prec<MIXED>;
   using R = typename P::R;
   using R2 = typename P::R2;
@@ -471,8 +479,9 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
         if (!MIXED && W <= nT) {
           // FP64 path (two waves per SIMD, 16-byte slots): the W bins are distinct, so each round
           // loads the touched slots' slab values before adding to any and waits for one memory
-          // latency instead of one per slot (8% faster there; the mixed path, with four waves per
-          // SIMD to hide the latency, ran 1.3% slower this way and keeps the slot-by-slot loop)
+          // latency instead of one per slot (8% faster there with rounds of 10; the mixed path,
+          // with four waves per SIMD to hide the latency, ran 1.3% slower this way and keeps the
+          // slot-by-slot loop)
           R2* row = (R2*)slab;
           for (int s0 = 0; s0 < W; s0 += kFlush) {
             R2 v[kFlush], old[kFlush];
@@ -511,10 +520,14 @@ __global__ __launch_bounds__(64) void tonal_eval_kernel(const TonalCaseDev* __re
         }
       }
     }
+    // the item's errors go to its case, so that a case whose window bound broke can be run again
+    const uint32_t flags = (wave_any(valid <= 0) ? kFlagNumeric : 0u) |
+                           (winmask ? kFlagWindow : 0u) | (sweepmask ? kFlagSweep : 0u);
+    if (flags && lane == 0) atomicOr(Cp->flags, flags);
+    valid = 0x7fffffff;
+    winmask = 0;
+    sweepmask = 0;
   }
-  const uint32_t flags = (wave_any(valid <= 0) ? kFlagNumeric : 0u) |
-                         (winmask ? kFlagWindow : 0u) | (sweepmask ? kFlagSweep : 0u);
-  if (flags && lane == 0) atomicOr(flags_out, flags);
 }
 
 // Ordered reduction of the partial slabs: out(b, o) = sum_g slab[g][o][b], FP64 Kahan.
@@ -720,16 +733,16 @@ __global__ void tonal_bins_kernel(const PosRec* __restrict__ pos, int64_t nE, Ti
 
 hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
                              const TonalCaseDev* cases, const Tile* tiles, const Item* items,
-                             const int* qoff, int* counter, uint32_t* flags) {
+                             const int* qoff, int* counter) {
   dim3 b(64);
 #define ACUM_EVAL(M, WW)                                                                       \
   if (mixed == M && W == WW) {                                                                 \
     if (moving)                                                                                \
       hipLaunchKernelGGL((tonal_eval_kernel<M, true, WW>), dim3(grid), b, 0, s, cases, tiles,  \
-                         items, qoff, counter, flags);                                       \
+                         items, qoff, counter);                                              \
     else                                                                                       \
       hipLaunchKernelGGL((tonal_eval_kernel<M, false, WW>), dim3(grid), b, 0, s, cases, tiles, \
-                         items, qoff, counter, flags);                                       \
+                         items, qoff, counter);                                              \
     return hipGetLastError();                                                                  \
   }
   ACUM_EVAL_VARIANTS(ACUM_EVAL)

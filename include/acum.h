@@ -120,6 +120,10 @@ typedef struct acum_tune {
   int32_t max_chunk;   /* most source time steps per window pass                              */
   int32_t waves;       /* waves per workgroup: always 1 (one 64-observer wave per workgroup;  */
                        /* the LDS window, not the workgroup size, sets the occupancy)         */
+  int32_t motion_permille; /* planning bound on observer-time change per metre of source    */
+                       /* motion, per mille of the worst case (0 = default 600); a case whose */
+                       /* window overflows under it is planned with the full bound and run    */
+                       /* again within the same call                                         */
 } acum_tune;
 
 /* --- context ----------------------------------------------------------------------------- */
