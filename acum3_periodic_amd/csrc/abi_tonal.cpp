@@ -859,6 +859,7 @@ int run_stage_once(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_gra
   for (int g : p.G) t.n_partials = std::max(t.n_partials, g);
   t.n_items = (int32_t)p.items.size();
   t.launches = 3 + (int32_t)st.cases.size();
+  for (auto& sc : st.cases) t.full_motion_cases += sc.full_motion ? 1 : 0;
   return check_flags(ctx, st, prec, retry);
 }
 

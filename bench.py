@@ -289,6 +289,7 @@ def bench_batched(acum, O, args):
     host_s = 0.0
     tuned = None
     checked = 0
+    full_motion = 0  # cases that needed the full source-motion bound
     for w0 in range(0, args.cases, args.wave):
         idx = list(range(w0, min(args.cases, w0 + args.wave)))
         t0 = time.time()
@@ -305,6 +306,7 @@ def bench_batched(acum, O, args):
         ctx.run(h, acum.MIXED, repeats=1, graph=True)  # plan, capture, warm
         tm = ctx.run(h, acum.MIXED, repeats=2, graph=True)
         gpu_ms += tm.total_ms
+        full_motion += tm.full_motion_cases
         ev = sum(sum(c.E[s] for s in range(c.nSurf)) * c.nTime * c.nObs for c in cases)
         total_evals += ev
         batch_out = ctx.fetch(h, 0, cases[0].nTime, cases[0].nObs)
@@ -325,7 +327,8 @@ def bench_batched(acum, O, args):
                       "tile": {"window": tuned.window, "max_spread": tuned.max_spread,
                                "max_tile": tuned.max_tile}},
            "gpu_s": round(gpu_ms / 1e3, 3), "host_synth_prep_s": round(host_s, 1),
-           "batches_checked_against_alone": checked}
+           "batches_checked_against_alone": checked,
+           "cases_on_full_motion_bound": full_motion}
     ctx.close()
     print(json.dumps(out))
     return 0
@@ -354,6 +357,7 @@ def bench_stress(acum, O, args):
     ms = []
     for _ in range(max(1, args.steps)):
         ms.append(ctx.run(h, acum.MIXED).total_ms)
+    full_motion = ctx.timing().full_motion_cases
     g = ctx.fetch(h, 0, case.nTime, case.nObs)
     ctx.release(h)
     nob = max(1, min(args.oracle_obs, case.nObs))
@@ -374,7 +378,7 @@ def bench_stress(acum, O, args):
            "n_gpus": 1, "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
            "config": {"workload": "coaxial 2 x 4 blades x 50,000 elements, 2048 steps, 4096 "
                                   "observers, Minf 0.15, impulsive loading", "evals": evals},
-           "ms_per_pass": round(step_ms, 1),
+           "ms_per_pass": round(step_ms, 1), "on_full_motion_bound": bool(full_motion),
            "accuracy": {"observers": nob, "rel_l2_pA": O.rel_l2(gA, oA),
                         "rel_l2_pT": O.rel_l2(g[0][:, :nob], oT), "rel_l2_pL": O.rel_l2(g[1][:, :nob], oL),
                         "max_doaspl_db": float(d.max())},
@@ -572,7 +576,8 @@ def main():
                    "evals_per_step": evals, "precision": "mixed", "graph": bool(args.graph),
                    "parallelism": "1 GPU",
                    "tile": {"window": tune.window, "max_spread": tune.max_spread,
-                            "max_tile": tune.max_tile}},
+                            "max_tile": tune.max_tile, "motion_permille": tune.motion_permille,
+                            "on_full_motion_bound": bool(tm.full_motion_cases)}},
         # compute-bound ("mfma" in the roofline vocabulary: the arithmetic peak, not HBM); the
         # arithmetic runs on the VALU (FP64 solve, FP32 integrand) and the matrix cores (the
         # integrand's record products)

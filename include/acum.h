@@ -106,7 +106,8 @@ typedef struct acum_timing {
   int32_t n_partials;    /* partial slabs per observer row (P)                               */
   int32_t n_items;       /* work items of the persistent launch                              */
   int32_t launches;      /* kernel launches of the run                                       */
-  int32_t reserved;
+  int32_t full_motion_cases; /* cases planned with the full source-motion bound (after a     */
+                             /* window overflow under the reduced one; see acum_tune)         */
 } acum_timing;
 
 /* Tile shapes of the tonal kernel (chosen by acum_autotune or set by the caller). */
