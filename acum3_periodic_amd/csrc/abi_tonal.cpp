@@ -1141,8 +1141,13 @@ int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* b
   const acum_tune keep = ctx->tune[precision];
   double bms = 1e300;
   acum_tune bt = keep;
+  auto fresh = [&]() {  // each shape starts from the reduced motion bound (its own fallback)
+    for (auto& sc : st->cases) sc.full_motion = false;
+    for (auto& p : st->plan) free_plan(p);
+  };
   for (const acum_tune& t : cand) {
     ctx->tune[precision] = t;
+    fresh();
     int rc = run_stage(ctx, *st, precision, 1, false);  // warm + plan
     if (rc == ACUM_OK) rc = run_stage(ctx, *st, precision, 2, false);
     if (rc == ACUM_OK && ctx->timing.total_ms < bms) {
@@ -1155,6 +1160,7 @@ int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* b
     return ctx_fail(ctx, ACUM_ERR_PLAN, "no candidate tile shape ran");
   }
   ctx->tune[precision] = bt;
+  fresh();
   if (best) *best = bt;
   if (best_ms) *best_ms = bms;
   return ACUM_OK;

@@ -199,3 +199,25 @@ def test_window_overflow_falls_back_to_full_motion_bound(ctx, prec):
         assert np.array_equal(x, y)
     o = O.tonal_f64(c)
     assert O.rel_l2(got[2], o[2]) <= (REL_L2_MIXED if prec == acum.MIXED else REL_L2_FP64)
+
+
+def test_autotune_picks_a_candidate_and_leaves_the_stage_consistent(ctx):
+    """acum_autotune times the candidate tile shapes on a staged case and keeps the fastest; a
+    later run of that stage equals a fresh stage run with the chosen shape set explicitly."""
+    keep = ctx.get_tuning(acum.MIXED)
+    try:
+        run, c = small_case(2, acum.MIXED)
+        h = ctx.stage([c])
+        best, ms = ctx.autotune(h, acum.MIXED)
+        assert best.window in (16, 20, 24, 32) and best.max_spread in (2, 3, 4) and ms > 0
+        ctx.run(h, acum.MIXED)
+        tuned = ctx.fetch(h, 0, c.nTime, c.nObs)
+        ctx.release(h)
+        ctx.set_tuning(acum.MIXED, window=best.window, max_spread=best.max_spread,
+                       max_tile=best.max_tile, motion_permille=best.motion_permille)
+        again = ctx.tonal(c)
+    finally:
+        ctx.set_tuning(acum.MIXED, window=keep.window, max_spread=keep.max_spread,
+                       max_tile=keep.max_tile, motion_permille=keep.motion_permille)
+    for x, y in zip(tuned, again):
+        assert np.array_equal(x, y)
