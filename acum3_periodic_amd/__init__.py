@@ -50,7 +50,7 @@ class Timing(C.Structure):
     _fields_ = [("prep_ms", C.c_double), ("eval_ms", C.c_double), ("reduce_ms", C.c_double),
                 ("total_ms", C.c_double), ("evals", C.c_int64), ("partial_bytes", C.c_int64),
                 ("n_partials", C.c_int32), ("n_items", C.c_int32), ("launches", C.c_int32),
-                ("full_motion_cases", C.c_int32)]
+                ("motion_widened_cases", C.c_int32)]
 
 
 class Tune(C.Structure):

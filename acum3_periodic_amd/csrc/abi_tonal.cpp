@@ -51,7 +51,8 @@ struct StagedCase {
   PosRec* d_pos = nullptr;                                  // shared by both precisions
   void* d_attr[2] = {nullptr, nullptr};                     // [ACUM_FP64], [ACUM_MIXED]
   float* d_out[3] = {nullptr, nullptr, nullptr};
-  bool full_motion = false;  // plan with the full source-motion bound (set after a window overflow)
+  int motion_level = 0;  // 0: the tuned source-motion bound; 1: at least 0.8; 2: the full bound
+                         // (raised by one after each window overflow)
 };
 
 // Default of acum_tune.motion_permille: on the four-blade rotor a bound of 0.6 holds for every
@@ -484,7 +485,8 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
 int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vector<Tile>& tiles) {
   const int64_t nT = sc.nT;
   // bins of observer time per metre of source motion: the full bound, or the reduced one
-  const double motion = sc.full_motion ? 1.0 : (t.motion_permille > 0 ? t.motion_permille : kMotionPermille) / 1000.0;
+  const int pm = t.motion_permille > 0 ? t.motion_permille : kMotionPermille;
+  const double motion = sc.motion_level >= 2 ? 1.0 : (sc.motion_level == 1 ? std::max(pm, 800) : pm) / 1000.0;
   const double L = sc.k.r2b / (1.0 - sc.Minf) * motion;
   const int W = t.window;
   const float* P = sc.pos.data();
@@ -750,8 +752,8 @@ int enqueue_pass(acum_ctx* ctx, Stage& st, int prec, bool with_events) {
   return ACUM_OK;
 }
 
-// Reads the cases' error flags after a pass.  A case planned with the reduced source-motion bound
-// whose window overflowed is marked for the full bound (*retry); any other error fails the run.
+// Reads the cases' error flags after a pass.  A case planned below the full source-motion bound
+// whose window overflowed moves to the next wider bound (*retry); any other error fails the run.
 int check_flags(acum_ctx* ctx, Stage& st, int prec, bool* retry) {
   const int nc = (int)st.cases.size();
   std::vector<uint32_t> f(nc, 0u);
@@ -759,8 +761,8 @@ int check_flags(acum_ctx* ctx, Stage& st, int prec, bool* retry) {
   *retry = false;
   uint32_t all = 0;
   for (int c = 0; c < nc; c++) {
-    if ((f[c] & kFlagWindow) && !st.cases[c].full_motion) {
-      st.cases[c].full_motion = true;
+    if ((f[c] & kFlagWindow) && st.cases[c].motion_level < 2) {
+      st.cases[c].motion_level++;
       *retry = true;
       f[c] &= ~kFlagWindow;
     }
@@ -859,22 +861,23 @@ int run_stage_once(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_gra
   for (int g : p.G) t.n_partials = std::max(t.n_partials, g);
   t.n_items = (int32_t)p.items.size();
   t.launches = 3 + (int32_t)st.cases.size();
-  for (auto& sc : st.cases) t.full_motion_cases += sc.full_motion ? 1 : 0;
+  for (auto& sc : st.cases) t.motion_widened_cases += sc.motion_level > 0 ? 1 : 0;
   return check_flags(ctx, st, prec, retry);
 }
 
 // Tiles are planned with a reduced bound on how far a source moves in observer time per step
 // (acum_tune.motion_permille), which lengthens the passes; the kernel checks every footprint
-// against its window, so a case whose bound was too tight is planned again with the full bound
-// and the pass is run again.  The plan of a case depends only on the case, so results agree
+// against its window, so a case whose bound was too tight is planned again with a wider bound
+// (at least 0.8, then the full one) and the pass is run again.  The plan of a case depends only on the case, so results agree
 // between a case run alone and in a batch.
 int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
   bool retry = false;
   int rc = run_stage_once(ctx, st, prec, repeats, use_graph, &retry);
-  if (rc || !retry) return rc;
-  for (auto& p : st.plan) free_plan(p);  // both precisions' plans hold the case's tiles
-  rc = run_stage_once(ctx, st, prec, repeats, use_graph, &retry);
-  if (!rc && retry) return ctx_fail(ctx, ACUM_ERR_PLAN, "a contribution fell outside its tile window");
+  for (int attempt = 0; rc == ACUM_OK && retry && attempt < 2; attempt++) {
+    for (auto& p : st.plan) free_plan(p);  // both precisions' plans hold the cases' tiles
+    rc = run_stage_once(ctx, st, prec, repeats, use_graph, &retry);
+  }
+  if (rc == ACUM_OK && retry) return ctx_fail(ctx, ACUM_ERR_PLAN, "a contribution fell outside its tile window");
   return rc;
 }
 
@@ -1142,7 +1145,7 @@ int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* b
   double bms = 1e300;
   acum_tune bt = keep;
   auto fresh = [&]() {  // each shape starts from the reduced motion bound (its own fallback)
-    for (auto& sc : st->cases) sc.full_motion = false;
+    for (auto& sc : st->cases) sc.motion_level = 0;
     for (auto& p : st->plan) free_plan(p);
   };
   for (const acum_tune& t : cand) {

@@ -106,8 +106,8 @@ typedef struct acum_timing {
   int32_t n_partials;    /* partial slabs per observer row (P)                               */
   int32_t n_items;       /* work items of the persistent launch                              */
   int32_t launches;      /* kernel launches of the run                                       */
-  int32_t full_motion_cases; /* cases planned with the full source-motion bound (after a     */
-                             /* window overflow under the reduced one; see acum_tune)         */
+  int32_t motion_widened_cases; /* cases planned with a wider source-motion bound than tuned */
+                                /* after a window overflow (see acum_tune)                    */
 } acum_timing;
 
 /* Tile shapes of the tonal kernel (chosen by acum_autotune or set by the caller). */
@@ -123,8 +123,8 @@ typedef struct acum_tune {
                        /* the LDS window, not the workgroup size, sets the occupancy)         */
   int32_t motion_permille; /* planning bound on observer-time change per metre of source    */
                        /* motion, per mille of the worst case (0 = default 600); a case whose */
-                       /* window overflows under it is planned with the full bound and run    */
-                       /* again within the same call                                         */
+                       /* window overflows under it is planned again with at least 800, then  */
+                       /* the full bound, and run again within the same call                  */
 } acum_tune;
 
 /* --- context ----------------------------------------------------------------------------- */
