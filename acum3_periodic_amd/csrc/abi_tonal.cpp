@@ -299,6 +299,8 @@ int validate(acum_ctx* ctx, const acum_tonal_case* c) {
     return ctx_fail(ctx, ACUM_ERR_ARG, "null array in case");
   for (int64_t s = 0; s < c->nSurf; s++)
     if (c->E[s] < 1) return ctx_fail(ctx, ACUM_ERR_ARG, "surface with no elements");
+  for (int64_t i = 0; i < 3 * c->nObs; i++)
+    if (!std::isfinite(c->rObs[i])) return ctx_fail(ctx, ACUM_ERR_ARG, "non-finite observer position");
   if (!(c->a0 > 0) || !(c->dPsi > 0) || !(c->dTau > 0) || !(c->oM > 0))
     return ctx_fail(ctx, ACUM_ERR_ARG, "a0, dPsi, dTau and oM must be positive");
   const double vv = c->vInf[0] * c->vInf[0] + c->vInf[1] * c->vInf[1] + c->vInf[2] * c->vInf[2];

@@ -84,7 +84,7 @@ constexpr int kQueues = 8;
 // Error flags raised by the kernels (OR-ed into a device word).
 enum : uint32_t {
   kFlagWindow = 1u,   // a contribution fell outside its tile window (planner bound broken)
-  kFlagNumeric = 2u,  // non-finite observer time, r <= 0 or 1 - Mr <= 0
+  kFlagNumeric = 2u,  // r (1 - Mr) <= 0, observer time not increasing, or a non-finite output
   kFlagSweep = 4u     // a bin sweep ran longer than the window
 };
 

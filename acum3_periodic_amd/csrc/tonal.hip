@@ -210,8 +210,9 @@ __device__ __forceinline__ float integrand_mfma(const MixRec& q, float dx, float
 }
 
 // The high word of a finite value read as a signed integer is positive exactly when the value is
-// positive and at least the smallest normal number: a running integer minimum of these words over
-// r (1 - Mr) and the observer-time steps flags every invalid eval with one instruction.
+// positive (for a double, at least 2^-1042): a running integer minimum of these words over r (1 - Mr) and the observer-time steps
+// flags every finite invalid eval with one instruction.  Non-finite values (a NaN reads positive)
+// are caught from the outputs by the reduction kernel.
 __device__ __forceinline__ int sign_word(float x) { return __float_as_int(x); }
 __device__ __forceinline__ int sign_word(double x) { return (int)(__double_as_longlong(x) >> 32); }
 
@@ -593,12 +594,23 @@ __global__ __launch_bounds__(256) void tonal_reduce_kernel(const TonalCaseDev* _
         add(2, (double)a1.x);
         add(3, (double)a1.y);
       }
-      sT[ol][2 * tx] = (float)s[0];
-      sL[ol][2 * tx] = (float)s[1];
-      sA[ol][2 * tx] = (float)(s[0] + s[1]);
-      sT[ol][2 * tx + 1] = (float)s[2];
-      sL[ol][2 * tx + 1] = (float)s[3];
-      sA[ol][2 * tx + 1] = (float)(s[2] + s[3]);
+      const float out[6] = {(float)s[0], (float)s[1], (float)(s[0] + s[1]),
+                            (float)s[2], (float)s[3], (float)(s[2] + s[3])};
+      sT[ol][2 * tx] = out[0];
+      sL[ol][2 * tx] = out[1];
+      sA[ol][2 * tx] = out[2];
+      sT[ol][2 * tx + 1] = out[3];
+      sL[ol][2 * tx + 1] = out[4];
+      sA[ol][2 * tx + 1] = out[5];
+      // A non-finite output fails the case: an eval whose observer time or integrand is not
+      // finite (a source on an observer makes the square root's argument zero) puts a NaN into
+      // every slot of its footprint, whatever the weights, and the ordered sums carry it here;
+      // a float overflow of a finite sum lands here too.  The eval kernel's sign test covers
+      // the finite invalid evals (1 - Mr <= 0, observer time not increasing).
+      bool bad = false;
+#pragma unroll
+      for (int k = 0; k < 6; k++) bad |= !__builtin_isfinite(out[k]);
+      if (bad) atomicOr(C.flags, kFlagNumeric);
     }
   }
   __syncthreads();

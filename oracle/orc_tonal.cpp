@@ -29,6 +29,11 @@
 // (tau, tau_next) with weight (tau_next - b)/(tau_next - tau), and bin b is stored at b mod nT.
 // The reference's radian-domain steps (npi shift, repeated tObs2 -= dPsi) select the same bins
 // in exact arithmetic; tests check the two forms agree on sampled cases.
+//
+// Error codes follow the library's contract (include/acum.h), not the reference, which computes
+// on with whatever the arithmetic gives: a non-finite observer position is ACUM_ERR_ARG; an eval
+// with r <= 0 (source on an observer), 1 - Mr <= 0, observer time that does not increase from
+// one step to the next, or a non-finite observer time or integrand is ACUM_ERR_NUMERIC.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -100,8 +105,10 @@ struct Eval {
 // rho = rhodot = 1 and U = V (core.cu:71-72,109-111), so Un - Vn = 0 exactly; the dotted normal
 // velocities are forced to zero (core.cu:117-118).  The terms that vanish identically (Mo,
 // Modot, LF2, the (Un-Vn) parts of Ma_n/Man_dot) add an exact zero and are left out.
+// Returns false for an eval the library reports as ACUM_ERR_NUMERIC: r <= 0, 1 - Mr <= 0 or a
+// non-finite result.
 template <typename R>
-inline void integrand(const float* X, const float* Nn, const float* Vv, const float* PR,
+inline bool integrand(const float* X, const float* Nn, const float* Vv, const float* PR,
                       double ox, double oy, double oz, double r_d, const TimeConst& k,
                       R rhoRef, R* pThick, R* pLoad) {
   const R a0 = (R)k.a0;
@@ -141,6 +148,7 @@ inline void integrand(const float* X, const float* Nn, const float* Vv, const fl
   const R LF1 = Lidot_r * T1 * ds + Li_r * Mdot_r * T2 * ds;          // core.cu:138
   const R LN = (Li_r - Li_m) * T3 * a0 * ds + Li_r * (Mr - M * M) * T4 * a0 * ds; // core.cu:141
   *pLoad = LN + LF1;                                                  // core.cu:154
+  return r_d > 0.0 && om > (R)0 && std::isfinite(*pThick) && std::isfinite(*pLoad);
 }
 
 // Tent binning of one eval (core.cu:166-231) in bin units: add(bin, w) for every rising bin,
@@ -176,6 +184,8 @@ inline bool bin_eval(const TimeConst& k, double tp, double tc, double tn, int64_
 template <typename R, typename Out>
 int tonal_impl(const acum_tonal_case* c, Out* pT, Out* pL, Out* pA, int nthreads) {
   if (!c || c->nTime < 2 || c->nObs < 1 || c->nSurf < 1) return ACUM_ERR_ARG;
+  for (int64_t i = 0; i < 3 * c->nObs; i++)
+    if (!std::isfinite(c->rObs[i])) return ACUM_ERR_ARG;
   const TimeConst k = make_time_const(c);
   const int64_t nT = c->nTime, nObs = c->nObs;
   int status = ACUM_OK;
@@ -209,10 +219,11 @@ int tonal_impl(const acum_tonal_case* c, Out* pT, Out* pL, Out* pA, int nthreads
             double tn = tt[mp];
             if (t == nT - 1) tn = tn + (double)nT;
             R pth, plo;
-            integrand<R>(Xe + 4 * t, Ne + 6 * t, Ve + 6 * t, Pe + 2 * t, ox, oy, oz, rr[t], k,
-                         (R)c->rhoRef, &pth, &plo);
+            ok = integrand<R>(Xe + 4 * t, Ne + 6 * t, Ve + 6 * t, Pe + 2 * t, ox, oy, oz, rr[t], k,
+                              (R)c->rhoRef, &pth, &plo);
+            ok = ok && tc > tp;  // observer time increases from step to step
             int64_t nd, nu, ic;
-            ok = bin_eval(k, tp, tc, tn, &nd, &nu, &ic, [&](int64_t b, double w) {
+            ok = ok && bin_eval(k, tp, tc, tn, &nd, &nu, &ic, [&](int64_t b, double w) {
               const R wr = (R)w;
               bT[b] += pth * wr;
               bL[b] += plo * wr;

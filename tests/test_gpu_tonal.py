@@ -8,12 +8,13 @@ import pytest
 
 import acum3_periodic_amd as acum
 import oracle as O
+from parity import report
 
 pytestmark = pytest.mark.gpu
 
 REL_L2_FP64 = 1e-6       # FP64 GPU vs FP64 oracle, each of pT, pL, pA
 OASPL_DB = 0.01          # mixed GPU vs FP64 oracle
-REL_L2_MIXED = 2e-3      # mixed GPU vs FP64 oracle on pA (FP32 integrands, FP32 tile partials)
+REL_L2_MIXED = 1e-5      # mixed GPU vs FP64 oracle on pA (FP32 integrands, FP32 tile partials)
 
 
 def small_case(config=1, precision=acum.FP64, **over):
@@ -63,8 +64,9 @@ def test_fp64_matches_oracle(ctx, name, config, over):
     run, c = small_case(config, acum.FP64, **over)
     gT, gL, gA = ctx.tonal(c)
     oT, oL, oA = O.tonal_f64(c)
-    for g, o, what in ((gT, oT, "pT"), (gL, oL, "pL"), (gA, oA, "pA")):
-        err = O.rel_l2(g, o)
+    errs = [O.rel_l2(g, o) for g, o in ((gT, oT), (gL, oL), (gA, oA))]
+    report(f"fp64_{name}", rel_l2_pT=errs[0], rel_l2_pL=errs[1], rel_l2_pA=errs[2])
+    for err, what in zip(errs, ("pT", "pL", "pA")):
         assert err <= REL_L2_FP64, f"{name} {what}: rel L2 {err:.3e}"
 
 
@@ -74,8 +76,10 @@ def test_mixed_matches_oracle(ctx, name, config, over):
     gT, gL, gA = ctx.tonal(c)
     oT, oL, oA = O.tonal_f64(c)
     err = O.rel_l2(gA, oA)
-    assert err <= REL_L2_MIXED, f"{name}: pA rel L2 {err:.3e}"
     d = np.abs(acum.oaspl(gA) - acum.oaspl(oA.astype(np.float32)))
+    report(f"mixed_{name}", rel_l2_pA=err, rel_l2_pT=O.rel_l2(gT, oT), rel_l2_pL=O.rel_l2(gL, oL),
+           max_doaspl_db=float(d.max()))
+    assert err <= REL_L2_MIXED, f"{name}: pA rel L2 {err:.3e}"
     assert d.max() <= OASPL_DB, f"{name}: OASPL differs by {d.max():.4f} dB"
 
 
@@ -121,7 +125,7 @@ def test_tile_shapes_agree(ctx, prec, window, spread, tile):
         got = ctx.tonal(c)[2]
     finally:
         ctx.set_tuning(prec, window=keep.window, max_spread=keep.max_spread, max_tile=keep.max_tile)
-    assert O.rel_l2(got, ref) < (1e-5 if prec == acum.MIXED else 1e-12)
+    assert O.rel_l2(got, ref) < (2e-6 if prec == acum.MIXED else 1e-12)
 
 
 def test_batch_mixes_still_and_moving_cases(ctx):
