@@ -122,6 +122,7 @@ def _load():
         "acum_tonal_fetch_spectra": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, _f32p, _f32p, _f32p]),
         "acum_run_stage": (C.c_int, [P, P, _i32p]),
         "acum_run_save_cache": (C.c_int, [P, C.c_char_p]),
+        "acum_tonal_spectra_noalloc": (C.c_int, [P, C.c_int64, C.c_int64, P, C.c_double, C.c_int32, _f32p, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -141,7 +142,7 @@ EXPORTS = ("acum_device_count", "acum_ctx_create", "acum_ctx_destroy", "acum_las
            "acum_run_bpm_case", "acum_run_write_tonal", "acum_run_write_bb", "acum_run_save",
            "acum_run_surface_raw", "acum_synth_defaults", "acum_synth_batch_case", "acum_synth",
            "acum_oaspl", "acum_tonal_spectra", "acum_tonal_fetch_spectra",
-           "acum_run_stage", "acum_run_save_cache")
+           "acum_run_stage", "acum_run_save_cache", "acum_tonal_spectra_noalloc")
 
 
 class AcumError(RuntimeError):
@@ -354,6 +355,13 @@ class Context:
         oa = np.empty(nObs, np.float32)
         self._check(lib.acum_tonal_fetch_spectra(self._h, handle, icase, fc.size, fptr(fc), fptr(spl), fptr(oa)))
         return spl, oa
+
+    def spectra_noalloc(self, d_pA, nTime, nObs, dt, fc, d_spl, d_oaspl):
+        """acum_tonal_spectra_noalloc: d_pA [nTime, nObs], d_spl [nObs, nBands] and d_oaspl [nObs]
+        are device addresses (ints); the call allocates nothing."""
+        fc = np.ascontiguousarray(fc, np.float32)
+        self._check(lib.acum_tonal_spectra_noalloc(self._h, nTime, nObs, C.c_void_p(d_pA), float(dt), fc.size,
+                                                   fptr(fc), C.c_void_p(d_spl), C.c_void_p(d_oaspl)))
 
     def bins(self, case, iobs):
         nE = sum(case.E[s] for s in range(case.nSurf))

@@ -4,10 +4,13 @@
 #include <string>
 #include <vector>
 #include "abi_internal.hpp"
+#include "spectra_types.hpp"
 
 namespace acum {
 hipError_t launch_spectra(hipStream_t s, const float* pA, int nT, int nObs, double* E, const int* lo, const int* hi,
                           int nBands, float* spl, float* oaspl);
+hipError_t launch_spectra_scratch(int device, hipStream_t s, const float* pA, int nT, int nObs,
+                                  const BandTable& tab, int nBands, float* spl, float* oaspl);
 
 namespace {
 
@@ -18,37 +21,48 @@ constexpr double kSixthUp = 1.122462048309373;     // 2^(1/6)
 
 // Bin range [lo_b, hi_b) of every band: bin k (frequency k df, df = 1 / (nT dt)) belongs to band
 // b when edge_b <= k df < edge_{b+1}.
-void spectra_band_bins(int64_t nT, double dt, int nBands, const float* fc, std::vector<int>& lo,
-                       std::vector<int>& hi) {
+// The edges are computed one at a time, so no buffer is needed (the allocation-free entry point
+// fills fixed arrays with it).
+void spectra_band_bins(int64_t nT, double dt, int nBands, const float* fc, int* lo, int* hi) {
   const int nbins = (int)(nT / 2);
   const double df = 1.0 / ((double)nT * dt);
-  std::vector<double> edge(nBands + 1);
-  edge[0] = (double)fc[0] * kSixthDown;
-  for (int b = 1; b < nBands; b++) edge[b] = std::sqrt((double)fc[b - 1] * (double)fc[b]);
-  edge[nBands] = (double)fc[nBands - 1] * kSixthUp;
-  lo.assign(nBands, 1);
-  hi.assign(nBands, 1);
+  auto edge = [&](int b) {
+    if (b == 0) return (double)fc[0] * kSixthDown;
+    if (b == nBands) return (double)fc[nBands - 1] * kSixthUp;
+    return std::sqrt((double)fc[b - 1] * (double)fc[b]);
+  };
   auto first_at_or_above = [&](double f) {
     int k = 1;
     while (k <= nbins && (double)k * df < f) k++;
     return k;
   };
   for (int b = 0; b < nBands; b++) {
-    lo[b] = first_at_or_above(edge[b]);
-    hi[b] = first_at_or_above(edge[b + 1]);
+    lo[b] = first_at_or_above(edge(b));
+    hi[b] = first_at_or_above(edge(b + 1));
   }
 }
 
-int spectra_device(acum_ctx* ctx, const float* d_pA, int64_t nT, int64_t nObs, double dt, int nBands,
-                   const float* fc, float* spl, float* oaspl) {
-  if (nT < 2 || nT > 65536 || nObs < 1 || nObs > (1 << 26) || nBands < 1 || !fc || !spl || !oaspl)
+namespace {
+
+int check_spectra_args(acum_ctx* ctx, int64_t nT, int64_t nObs, double dt, int nBands, const float* fc) {
+  if (nT < 2 || nT > 65536 || nObs < 1 || nObs > (1 << 26) || nBands < 1 || !fc)
     return ctx_fail(ctx, ACUM_ERR_ARG, "spectra need 2 <= nTime <= 65536, nObs >= 1, nBands >= 1 and outputs");
   if (!(dt > 0) || !std::isfinite(dt)) return ctx_fail(ctx, ACUM_ERR_ARG, "dt must be positive");
   for (int b = 0; b < nBands; b++)
     if (!(fc[b] > 0) || !std::isfinite(fc[b]) || (b > 0 && !(fc[b] > fc[b - 1])))
       return ctx_fail(ctx, ACUM_ERR_ARG, "band centres must be positive and strictly increasing");
-  std::vector<int> lo, hi;
-  spectra_band_bins(nT, dt, nBands, fc, lo, hi);
+  return ACUM_OK;
+}
+
+}  // namespace
+
+int spectra_device(acum_ctx* ctx, const float* d_pA, int64_t nT, int64_t nObs, double dt, int nBands,
+                   const float* fc, float* spl, float* oaspl) {
+  if (!spl || !oaspl) return ctx_fail(ctx, ACUM_ERR_ARG, "null spectra output");
+  int rc = check_spectra_args(ctx, nT, nObs, dt, nBands, fc);
+  if (rc) return rc;
+  std::vector<int> lo(nBands), hi(nBands);
+  spectra_band_bins(nT, dt, nBands, fc, lo.data(), hi.data());
   const int nbins = (int)(nT / 2);
   hipStream_t s = ctx_stream(ctx);
   double* d_E;
@@ -110,4 +124,38 @@ extern "C" int acum_tonal_spectra(acum_ctx* ctx, int64_t nTime, int64_t nObs, co
                              : ctx_fail(ctx, ACUM_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
   (void)hipFree(d_pA);
   return rc;
+}
+
+// Allocation-free form: device pA, device outputs, the library's fixed device scratch.
+extern "C" int acum_tonal_spectra_noalloc(acum_ctx* ctx, int64_t nTime, int64_t nObs, const float* d_pA, double dt,
+                                          int32_t nBands, const float* fc, float* d_spl, float* d_oaspl) {
+  if (!ctx) return ACUM_ERR_ARG;
+  if (!d_pA || !d_spl || !d_oaspl) return ctx_fail(ctx, ACUM_ERR_ARG, "null device pointer");
+  int rc = check_spectra_args(ctx, nTime, nObs, dt, nBands, fc);
+  if (rc) return rc;
+  if (nBands > kMaxBandsNoAlloc)
+    return ctx_fail(ctx, ACUM_ERR_ARG, "acum_tonal_spectra_noalloc takes at most 128 bands");
+  const size_t need = (size_t)(nTime / 2 + 1) * (size_t)nObs * sizeof(double);
+  if (need > kSpectraScratchBytes)
+    return ctx_fail(ctx, ACUM_ERR_ARG,
+                    "acum_tonal_spectra_noalloc: (nTime/2 + 1) * nObs doubles exceed its 64 MiB device scratch");
+  BandTable tab;
+  spectra_band_bins(nTime, dt, nBands, fc, tab.lo, tab.hi);
+  if (hipSetDevice(ctx_device(ctx)) != hipSuccess) return ctx_fail(ctx, ACUM_ERR_DEVICE, "hipSetDevice failed");
+  hipStream_t s = ctx_stream(ctx);
+  hipError_t e = hipEventRecord(ctx_event(ctx, 0), s);
+  if (e == hipSuccess)
+    e = launch_spectra_scratch(ctx_device(ctx), s, d_pA, (int)nTime, (int)nObs, tab, nBands, d_spl, d_oaspl);
+  if (e == hipSuccess) e = hipEventRecord(ctx_event(ctx, 1), s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx_event(ctx, 0), ctx_event(ctx, 1));
+  if (e != hipSuccess) return ctx_fail(ctx, ACUM_ERR_DEVICE, std::string("spectra (no allocation): ") + hipGetErrorString(e));
+  acum_timing* t = ctx_timing(ctx);
+  *t = acum_timing();
+  t->eval_ms = ms;
+  t->total_ms = ms;
+  t->evals = (int64_t)(nTime / 2) * nTime * nObs;
+  t->launches = 3;
+  return ACUM_OK;
 }

@@ -33,6 +33,7 @@ hipError_t launch_reset(hipStream_t s, int* counter, uint32_t* flags);
 hipError_t launch_tonal_bins(hipStream_t s, const void* pos, int64_t nE, const TimeConst& k,
                              double ox, double oy, double oz, int32_t* ipts,
                              int32_t* npi, int32_t* nd, int32_t* nu);
+hipError_t spectra_scratch_init(int device);
 }  // namespace acum
 
 using namespace acum;
@@ -937,6 +938,11 @@ int acum_ctx_create(acum_ctx** out, int device) {
     return ACUM_ERR_DEVICE;
   }
   c->cu_count = prop.multiProcessorCount;
+  // the device address of this GPU's scratch array of acum_tonal_spectra_noalloc
+  if (spectra_scratch_init(device) != hipSuccess) {
+    acum_ctx_destroy(c);
+    return ACUM_ERR_DEVICE;
+  }
   c->tune[ACUM_FP64] = default_tune(ACUM_FP64);
   c->tune[ACUM_MIXED] = default_tune(ACUM_MIXED);
   *out = c;

@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
+#include "spectra_types.hpp"
 
 namespace acum {
 namespace {
@@ -131,13 +133,24 @@ __global__ __launch_bounds__(64) void spectra_bins_kernel(const float* __restric
   }
 }
 
-__global__ __launch_bounds__(64) void spectra_bands_kernel(const double* __restrict__ E, int nObs, const int* __restrict__ lo,
-                                                           const int* __restrict__ hi, int nBands,
-                                                           float* __restrict__ spl) {
+// The band table [lo_b, hi_b) comes from device memory (BandPtrs) or as a kernel argument
+// (BandTable, the allocation-free entry point); the sums are the same either way.
+struct BandPtrs {
+  const int* lo;
+  const int* hi;
+};
+__device__ __forceinline__ int band_lo(const BandPtrs& t, int b) { return t.lo[b]; }
+__device__ __forceinline__ int band_hi(const BandPtrs& t, int b) { return t.hi[b]; }
+__device__ __forceinline__ int band_lo(const BandTable& t, int b) { return t.lo[b]; }
+__device__ __forceinline__ int band_hi(const BandTable& t, int b) { return t.hi[b]; }
+
+template <typename Table>
+__global__ __launch_bounds__(64) void spectra_bands_kernel(const double* __restrict__ E, int nObs, Table tab,
+                                                           int nBands, float* __restrict__ spl) {
   const int o = blockIdx.x * 64 + threadIdx.x;
   if (o >= nObs) return;
   for (int b = blockIdx.y; b < nBands; b += gridDim.y) {
-    const int k0 = lo[b], k1 = hi[b];
+    const int k0 = band_lo(tab, b), k1 = band_hi(tab, b);
     double e = 0.0;
     int k = k0;
     for (; k + kChunk <= k1; k += kChunk) {
@@ -152,11 +165,9 @@ __global__ __launch_bounds__(64) void spectra_bands_kernel(const double* __restr
   }
 }
 
-}  // namespace
-
-// E holds nbins * nObs bin energies followed by nObs observer means.
-hipError_t launch_spectra(hipStream_t s, const float* pA, int nT, int nObs, double* E, const int* lo, const int* hi,
-                          int nBands, float* spl, float* oaspl) {
+template <typename Table>
+hipError_t launch_all(hipStream_t s, const float* pA, int nT, int nObs, double* E, const Table& tab, int nBands,
+                      float* spl, float* oaspl) {
   const int nbins = nT / 2;
   const int og = (nObs + 63) / 64;
   double* mean = E + (int64_t)nbins * nObs;
@@ -166,8 +177,53 @@ hipError_t launch_spectra(hipStream_t s, const float* pA, int nT, int nObs, doub
     hipLaunchKernelGGL(spectra_bins_kernel, g, dim3(64), 0, s, pA, mean, nT, nObs, nbins, E);
   }
   const dim3 gb(og, nBands < 65535 ? nBands : 65535);
-  hipLaunchKernelGGL(spectra_bands_kernel, gb, dim3(64), 0, s, E, nObs, lo, hi, nBands, spl);
+  hipLaunchKernelGGL(spectra_bands_kernel<Table>, gb, dim3(64), 0, s, E, nObs, tab, nBands, spl);
   return hipGetLastError();
+}
+
+// Scratch of the allocation-free entry point (acum_tonal_spectra_noalloc): the bin energies and
+// observer means of one call, kSpectraScratchBytes per GPU, held from the library's first
+// context on that GPU until the process ends.  Host code never uses this name as an address:
+// the device address of each GPU's copy comes from hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[kSpectraScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+}  // namespace
+
+// E holds nbins * nObs bin energies followed by nObs observer means.
+hipError_t launch_spectra(hipStream_t s, const float* pA, int nT, int nObs, double* E, const int* lo, const int* hi,
+                          int nBands, float* spl, float* oaspl) {
+  return launch_all(s, pA, nT, nObs, E, BandPtrs{lo, hi}, nBands, spl, oaspl);
+}
+
+// Resolves the device address of the scratch array on `device`, which must be the current
+// device; called by acum_ctx_create for each context's device.
+hipError_t spectra_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch[device] = p;
+  return hipSuccess;
+}
+
+// The same kernels on the scratch array of `device` (resolved by spectra_scratch_init) with the
+// band table passed by value: no allocation and no copy.  The caller has checked that
+// (nT / 2 + 1) nObs doubles fit kSpectraScratchBytes and that nBands <= kMaxBandsNoAlloc.
+hipError_t launch_spectra_scratch(int device, hipStream_t s, const float* pA, int nT, int nObs,
+                                  const BandTable& tab, int nBands, float* spl, float* oaspl) {
+  void* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device >= 0 && device < kMaxDevices) base = g_scratch[device];
+  }
+  if (!base) return hipErrorNotInitialized;
+  return launch_all(s, pA, nT, nObs, (double*)base, tab, nBands, spl, oaspl);
 }
 
 }  // namespace acum

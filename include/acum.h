@@ -252,6 +252,19 @@ int acum_tonal_spectra(acum_ctx* ctx, int64_t nTime, int64_t nObs, const float* 
 int acum_tonal_fetch_spectra(acum_ctx* ctx, int32_t handle, int32_t icase, int32_t nBands,
                              const float* fc, float* spl, float* oaspl);
 
+/* Allocation-free form of acum_tonal_spectra.  No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc).  pA [nTime][nObs], spl [nObs][nBands] and oaspl [nObs] are DEVICE pointers; fc is a
+ * host array.  The call allocates nothing, host or device: the bin energies and observer means go
+ * to a fixed device scratch array of 64 MiB per GPU, held from the first context on that GPU until
+ * the process ends, and the band table travels as a kernel argument.  Limits: nBands <= 128 and
+ * (nTime/2 + 1) * nObs * 8 bytes <= 64 MiB (e.g. 2048 steps x 8000 observers); beyond them the
+ * call returns ACUM_ERR_ARG.  Calls that overlap on one GPU (from several contexts or threads)
+ * race on the shared scratch; calls on one context are serialised.  Results are bit-identical
+ * to acum_tonal_spectra's.                                                                    */
+int acum_tonal_spectra_noalloc(acum_ctx* ctx, int64_t nTime, int64_t nObs, const float* pA, double dt,
+                               int32_t nBands, const float* fc, float* spl, float* oaspl);
+
 #ifdef __cplusplus
 }
 #endif
