@@ -50,7 +50,8 @@ class Timing(C.Structure):
     _fields_ = [("prep_ms", C.c_double), ("eval_ms", C.c_double), ("reduce_ms", C.c_double),
                 ("total_ms", C.c_double), ("evals", C.c_int64), ("partial_bytes", C.c_int64),
                 ("n_partials", C.c_int32), ("n_items", C.c_int32), ("launches", C.c_int32),
-                ("motion_widened_cases", C.c_int32)]
+                ("motion_widened_cases", C.c_int32), ("eval_launches", C.c_int32),
+                ("tune_cases", C.c_int32), ("tune_hits", C.c_int32), ("tune_passes", C.c_int32)]
 
 
 class Tune(C.Structure):
@@ -123,6 +124,10 @@ def _load():
         "acum_run_stage": (C.c_int, [P, P, _i32p]),
         "acum_run_save_cache": (C.c_int, [P, C.c_char_p]),
         "acum_tonal_spectra_noalloc": (C.c_int, [P, C.c_int64, C.c_int64, P, C.c_double, C.c_int32, _f32p, P, P]),
+        "acum_get_case_tuning": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Tune)]),
+        "acum_tune_cache_save": (C.c_int, [P, C.c_char_p]),
+        "acum_tune_cache_load": (C.c_int, [P, C.c_char_p]),
+        "acum_tune_cache_clear": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -142,7 +147,8 @@ EXPORTS = ("acum_device_count", "acum_ctx_create", "acum_ctx_destroy", "acum_las
            "acum_run_bpm_case", "acum_run_write_tonal", "acum_run_write_bb", "acum_run_save",
            "acum_run_surface_raw", "acum_synth_defaults", "acum_synth_batch_case", "acum_synth",
            "acum_oaspl", "acum_tonal_spectra", "acum_tonal_fetch_spectra",
-           "acum_run_stage", "acum_run_save_cache", "acum_tonal_spectra_noalloc")
+           "acum_run_stage", "acum_run_save_cache", "acum_tonal_spectra_noalloc", "acum_get_case_tuning",
+           "acum_tune_cache_save", "acum_tune_cache_load", "acum_tune_cache_clear")
 
 
 class AcumError(RuntimeError):
@@ -388,10 +394,26 @@ class Context:
         self._check(lib.acum_set_tuning(self._h, precision, C.byref(t)))
 
     def autotune(self, handle, precision):
+        """Tunes the buckets of the staged cases not yet in the context's cache; returns case 0's
+        shape and its time per pass on the tuning subset."""
         t = Tune()
         ms = C.c_double()
         self._check(lib.acum_autotune(self._h, handle, precision, C.byref(t), C.byref(ms)))
         return t, ms.value
+
+    def case_tuning(self, handle, icase, precision):
+        t = Tune()
+        self._check(lib.acum_get_case_tuning(self._h, handle, icase, precision, C.byref(t)))
+        return t
+
+    def tune_cache_save(self, path):
+        self._check(lib.acum_tune_cache_save(self._h, str(path).encode()))
+
+    def tune_cache_load(self, path):
+        self._check(lib.acum_tune_cache_load(self._h, str(path).encode()))
+
+    def tune_cache_clear(self):
+        self._check(lib.acum_tune_cache_clear(self._h))
 
     def bpm(self, case):
         spl = np.full(case.nObs * case.nf2 * 7, -100.0, np.float32)

@@ -10,6 +10,7 @@
 #include <memory>
 #include <omp.h>
 #include <string>
+#include <tuple>
 #include <vector>
 #include "abi_internal.hpp"
 #include "host/run.hpp"
@@ -46,7 +47,10 @@ struct StagedCase {
   TimeConst k{};
   double rhoRef = 0, Minf = 0;
   std::vector<int64_t> surf0;  // first element of each surface
-  std::vector<float> pos;      // host positions [nE][nT][3], for planning
+  // host positions [nE][nT][3] and per element the largest source displacement between two
+  // steps, for planning; shared with the autotuner's views of the case
+  std::shared_ptr<std::vector<float>> pos;
+  std::shared_ptr<std::vector<double>> step;
   float* d_obs = nullptr;
   float* d_aos[4] = {nullptr, nullptr, nullptr, nullptr};  // X, N, V, PR
   PosRec* d_pos = nullptr;                                  // shared by both precisions
@@ -54,33 +58,56 @@ struct StagedCase {
   float* d_out[3] = {nullptr, nullptr, nullptr};
   int motion_level = 0;  // 0: the tuned source-motion bound; 1: at least 0.8; 2: the full bound
                          // (raised by one after each window overflow)
+  bool forced = false;   // plan with `force` (the autotuner's candidates) instead of the
+  acum_tune force{};     // context's shape for this case's bucket
 };
 
 // Default of acum_tune.motion_permille: on the four-blade rotor a bound of 0.6 holds for every
 // tile and observer (0.45 overflows) and lengthens the passes from ~5 to ~6 steps.
 constexpr int kMotionPermille = 600;
 
+// One eval launch: the work items of the cases that use one kernel variant (still air or moving
+// medium, window size), drained through kQueues queues; queue q holds items
+// [qoff[q], qoff[q+1]) of the launch's item range.
+struct Launch {
+  bool moving = false;
+  int window = 0;
+  int item0 = 0, nitems = 0;
+  int grid = 0;
+  int qoff[kQueues + 1] = {};
+};
+
 struct Plan {
   bool valid = false;
-  acum_tune tune{};
+  std::vector<acum_tune> tunes;  // the tile shape each case was planned with
   std::vector<Tile> tiles;
   std::vector<Item> items;
   std::vector<int> G;
   std::vector<int64_t> slab_off;
   int64_t slab_bytes = 0;
+  std::vector<Launch> launches;
   Tile* d_tiles = nullptr;
   Item* d_items = nullptr;
   TonalCaseDev* d_cases = nullptr;
   void* d_slab = nullptr;
   uint32_t* d_flags = nullptr;  // [cases] error flags of the last pass
-  int n_still = 0;  // items[0, n_still) are still-air cases, the rest moving-medium ones
-  int grid[2] = {0, 0};
-  // queue q of launch l (0 still air, 1 moving) holds items [qoff[l][q], qoff[l][q+1]) of that
-  // launch's item range
-  int qoff[2][kQueues + 1] = {};
-  int* d_qoff = nullptr;
+  int* d_qoff = nullptr;        // [launches][kQueues + 1]
+  int* d_counter = nullptr;     // [launches][kQueues] queue heads, cleared before each pass
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+};
+
+// Shape bucket of the autotuner's cache: precision, still air or moving medium, the rounded-up
+// base-2 logarithms of the element, step and observer counts, and the device's compute units.
+struct TuneKey {
+  int prec, moving, lgE, lgT, lgO, cus;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(prec, moving, lgE, lgT, lgO, cus) < std::tie(o.prec, o.moving, o.lgE, o.lgT, o.lgO, o.cus);
+  }
+};
+struct TuneEntry {
+  acum_tune tune;
+  double ms;  // time per pass of the chosen shape on the observer subset it was tuned on
 };
 
 struct Stage {
@@ -102,6 +129,7 @@ struct acum_ctx {
   int cu_count = 0;
   std::map<int, std::unique_ptr<Stage>> stages;
   int next_handle = 1;
+  std::map<TuneKey, TuneEntry> tune_cache;  // autotuned shapes by bucket (acum_autotune)
   void* ws = nullptr;  // per-call workspace (ctx_workspace)
   size_t ws_bytes = 0;
   void* pin[2] = {nullptr, nullptr};  // pinned staging buffers (ctx_upload / ctx_download)
@@ -246,6 +274,7 @@ void free_plan(Plan& p) {
   (void)hipFree(p.d_tiles);
   (void)hipFree(p.d_items);
   (void)hipFree(p.d_qoff);
+  (void)hipFree(p.d_counter);
   (void)hipFree(p.d_cases);
   (void)hipFree(p.d_slab);
   (void)hipFree(p.d_flags);
@@ -341,14 +370,14 @@ int stage_case(acum_ctx* ctx, const acum_tonal_case* c, StagedCase& sc) {
       e0 += c->E[s];
     }
   }
-  sc.pos.resize((size_t)q * 3);
+  sc.pos = std::make_shared<std::vector<float>>((size_t)q * 3);
   {
     int64_t e0 = 0;
     for (int64_t s = 0; s < c->nSurf; s++) {
       const float* X = c->X + c->XPtr[s];
 #pragma omp parallel for schedule(static)
       for (int64_t i = 0; i < c->E[s] * nT; i++)
-        for (int d = 0; d < 3; d++) sc.pos[(e0 * nT + i) * 3 + d] = X[i * 4 + d];
+        for (int d = 0; d < 3; d++) (*sc.pos)[(e0 * nT + i) * 3 + d] = X[i * 4 + d];
       e0 += c->E[s];
     }
   }
@@ -430,11 +459,11 @@ int stage_raw_run(acum_ctx* ctx, const acum_run* r, StagedCase& sc) {
   HIPCHK(ctx, launch_preprocess(ctx->stream, pa, (int)std::min<int64_t>(nE, 1 << 20)));
   // host positions for the planner: the preprocessing scales them by CFDscale in FP64 and rounds
   // to float, which the host repeats exactly
-  sc.pos.resize((size_t)q * 3);
+  sc.pos = std::make_shared<std::vector<float>>((size_t)q * 3);
   const double C = r->in.CFDscale;
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < q; i++)
-    for (int d = 0; d < 3; d++) sc.pos[i * 3 + d] = (float)((double)c.X[i * 4 + d] * C);
+    for (int d = 0; d < 3; d++) (*sc.pos)[i * 3 + d] = (float)((double)c.X[i * 4 + d] * C);
   HIPCHK(ctx, hipMalloc(&sc.d_obs, (size_t)c.nObs * 3 * sizeof(float)));
   HIPCHK(ctx, hipMemcpyAsync(sc.d_obs, c.rObs, (size_t)c.nObs * 3 * sizeof(float), hipMemcpyHostToDevice,
                              ctx->stream));
@@ -477,7 +506,32 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
   return ACUM_OK;
 }
 
-// Tiles of one case: runs of consecutive elements of one surface whose observer times stay within
+constexpr int64_t kPlanChunk = 4096;  // elements planned together (tiles never cross a chunk)
+
+// Largest source displacement between consecutive steps, per element (planning input, computed
+// once per staged case).
+void element_steps(StagedCase& sc) {
+  if (sc.step) return;
+  const int64_t nT = sc.nT;
+  const float* P = sc.pos->data();
+  auto st = std::make_shared<std::vector<double>>(sc.nE);
+  std::vector<double>& step = *st;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < sc.nE; e++) {
+    double m = 0;
+    for (int64_t k = 0; k < nT; k++) {
+      const float* a = P + (e * nT + k) * 3;
+      const float* b = P + (e * nT + (k + 1) % nT) * 3;
+      const double dx = (double)a[0] - b[0], dy = (double)a[1] - b[1], dz = (double)a[2] - b[2];
+      m = std::max(m, dx * dx + dy * dy + dz * dz);
+    }
+    step[e] = std::sqrt(m);
+  }
+  sc.step = st;
+}
+
+// Tiles of one case: runs of consecutive elements of one surface (within chunks of kPlanChunk
+// elements, planned in parallel) whose observer times stay within
 // `spread` - 1 bins of the first element's at every step (Lipschitz bound of the convected
 // retarded time: |dtau| <= |dx| L, L = oM / (a0 dPsi (1 - Minf)) bins per metre), with the pass
 // length K chosen so every bin a pass can touch fits the window.  With the window based at
@@ -485,40 +539,45 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
 //   lowest bin  >= floor(tau_e(k0)) - 1 >= base                       (footprint ic - 1)
 //   highest bin <= tau_e0(k0 - 1) + max(K step_bins + 2, (K + 1) step_bins) + spread
 // so K step_bins + 2 spread + 5 <= W and (K + 1) step_bins + 2 spread + 3 <= W.
-int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vector<Tile>& tiles) {
+int plan_tiles(acum_ctx* ctx, StagedCase& sc, const acum_tune& t, std::vector<Tile>& tiles) {
   const int64_t nT = sc.nT;
   // bins of observer time per metre of source motion: the full bound, or the reduced one
   const int pm = t.motion_permille > 0 ? t.motion_permille : kMotionPermille;
   const double motion = sc.motion_level >= 2 ? 1.0 : (sc.motion_level == 1 ? std::max(pm, 800) : pm) / 1000.0;
   const double L = sc.k.r2b / (1.0 - sc.Minf) * motion;
   const int W = t.window;
-  const float* P = sc.pos.data();
+  const float* P = sc.pos->data();
   auto dist2 = [&](int64_t e1, int64_t k1, int64_t e2, int64_t k2) {
     const float* a = P + (e1 * nT + k1) * 3;
     const float* b = P + (e2 * nT + k2) * 3;
     const double dx = (double)a[0] - b[0], dy = (double)a[1] - b[1], dz = (double)a[2] - b[2];
     return dx * dx + dy * dy + dz * dz;
   };
-  std::vector<double> step(sc.nE);
-#pragma omp parallel for schedule(static)
-  for (int64_t e = 0; e < sc.nE; e++) {
-    double m = 0;
-    for (int64_t k = 0; k < nT; k++) m = std::max(m, dist2(e, k, e, (k + 1) % nT));
-    step[e] = std::sqrt(m);
-  }
+  element_steps(sc);
+  const std::vector<double>& step = *sc.step;
   auto pass_len = [&](int spread, double S) {
     const double sb = sc.k.SB + S * L;
     const int K = (int)std::min(std::floor((W - 2.0 * spread - 5.0) / sb),
                                 std::floor((W - 2.0 * spread - 3.0) / sb) - 1);
     return std::min<int64_t>(std::min<int64_t>(K, t.max_chunk), nT);
   };
+  // chunks of at most kPlanChunk elements of one surface, planned in parallel; the chunking
+  // depends only on the case, so the tiles do not depend on the host's thread count
+  struct Range {
+    int64_t a, b;
+  };
+  std::vector<Range> chunks;
   const size_t nsurf = sc.surf0.size();
-  std::vector<std::vector<Tile>> per(nsurf);
-  int status = ACUM_OK;
-#pragma omp parallel for schedule(dynamic, 1)
   for (size_t s = 0; s < nsurf; s++) {
     const int64_t a = sc.surf0[s], b = (s + 1 < nsurf) ? sc.surf0[s + 1] : sc.nE;
-    int64_t e0 = a;
+    for (int64_t c = a; c < b; c += kPlanChunk) chunks.push_back(Range{c, std::min(b, c + kPlanChunk)});
+  }
+  std::vector<std::vector<Tile>> per(chunks.size());
+  int status = ACUM_OK;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (size_t ci = 0; ci < chunks.size(); ci++) {
+    const int64_t b = chunks[ci].b;
+    int64_t e0 = chunks[ci].a;
     while (e0 < b) {
       double R = 0, S = step[e0];
       int spread = 1;
@@ -546,7 +605,7 @@ int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vec
       tl.ne = (int32_t)(e - e0);
       tl.spread = spread;
       tl.K = K;
-      per[s].push_back(tl);
+      per[ci].push_back(tl);
       e0 = e;
     }
   }
@@ -556,27 +615,64 @@ int plan_tiles(acum_ctx* ctx, const StagedCase& sc, const acum_tune& t, std::vec
   return ACUM_OK;
 }
 
+// The tile shape a case is planned with: the autotuner's candidate while it times one, else the
+// cached shape of the case's bucket, else the context's shape for the precision.
+TuneKey tune_key(const acum_ctx* ctx, const StagedCase& sc, int prec) {
+  auto lg = [](int64_t n) {
+    int l = 0;
+    while (((int64_t)1 << l) < n) l++;
+    return l;
+  };
+  return TuneKey{prec, sc.k.still ? 0 : 1, lg(sc.nE), lg(sc.nT), lg(sc.nObs), ctx->cu_count};
+}
+
+acum_tune case_tune(const acum_ctx* ctx, const StagedCase& sc, int prec) {
+  if (sc.forced) return sc.force;
+  auto it = ctx->tune_cache.find(tune_key(ctx, sc, prec));
+  return it != ctx->tune_cache.end() ? it->second.tune : ctx->tune[prec];
+}
+
 int build_plan(acum_ctx* ctx, Stage& st, int prec) {
   Plan& p = st.plan[prec];
-  if (p.valid && std::memcmp(&p.tune, &ctx->tune[prec], sizeof(acum_tune)) == 0) return ACUM_OK;
-  free_plan(p);
-  p.tune = ctx->tune[prec];
-  const bool mixed = prec == ACUM_MIXED;
-  if (!eval_window_supported(mixed, p.tune.window))
-    return ctx_fail(ctx, ACUM_ERR_ARG, "no eval kernel for this window size");
-  bool any_moving = false;
-  for (auto& sc : st.cases) any_moving = any_moving || !sc.k.still;
-  int bpc = 0;
-  HIPCHK(ctx, eval_occupancy(mixed, any_moving, p.tune.window, &bpc));
-  if (bpc < 1) return ctx_fail(ctx, ACUM_ERR_DEVICE, "eval kernel does not fit a compute unit");
-  const int capacity = bpc * ctx->cu_count;
   const int nc = (int)st.cases.size();
+  std::vector<acum_tune> tunes(nc);
+  for (int c = 0; c < nc; c++) tunes[c] = case_tune(ctx, st.cases[c], prec);
+  if (p.valid && p.tunes.size() == tunes.size() &&
+      std::memcmp(p.tunes.data(), tunes.data(), nc * sizeof(acum_tune)) == 0)
+    return ACUM_OK;
+  free_plan(p);
+  p.tunes = tunes;
+  const bool mixed = prec == ACUM_MIXED;
+  // one launch per kernel variant (still air or moving medium, window) in use; a case's
+  // capacity is that of its own variant, so it gets the same groups alone or in a batch
+  std::vector<std::pair<int, int>> kinds;  // (moving, window)
+  std::vector<int> kind_of(nc), capacity(nc);
+  for (int c = 0; c < nc; c++) {
+    const StagedCase& sc = st.cases[c];
+    const int W = tunes[c].window;
+    if (!eval_window_supported(mixed, W)) return ctx_fail(ctx, ACUM_ERR_ARG, "no eval kernel for this window size");
+    int bpc = 0;
+    HIPCHK(ctx, eval_occupancy(mixed, !sc.k.still, W, &bpc));
+    if (bpc < 1) return ctx_fail(ctx, ACUM_ERR_DEVICE, "eval kernel does not fit a compute unit");
+    capacity[c] = bpc * ctx->cu_count;
+    kinds.emplace_back(sc.k.still ? 0 : 1, W);
+  }
+  {
+    std::vector<std::pair<int, int>> u = kinds;
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    for (int c = 0; c < nc; c++) kind_of[c] = (int)(std::lower_bound(u.begin(), u.end(), kinds[c]) - u.begin());
+    p.launches.assign(u.size(), Launch());
+    for (size_t l = 0; l < u.size(); l++) {
+      p.launches[l].moving = u[l].first != 0;
+      p.launches[l].window = u[l].second;
+    }
+  }
   std::vector<std::vector<Tile>> ct(nc);
   for (int c = 0; c < nc; c++) {
-    int rc = plan_tiles(ctx, st.cases[c], p.tune, ct[c]);
+    int rc = plan_tiles(ctx, st.cases[c], tunes[c], ct[c]);
     if (rc) return rc;
   }
-  const double target_items = 4.0 * capacity;
   const size_t asz = mixed ? sizeof(float) : sizeof(double);
   p.G.assign(nc, 1);
   p.slab_off.assign(nc, 0);
@@ -586,7 +682,8 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     const int nog = (int)((sc.nObs + 63) / 64);
     // G depends on this case alone, so a case gets the same tiles, groups and summation order
     // whether it runs alone or inside a batch (bit-identical results).
-    int G = p.tune.groups > 0 ? p.tune.groups : (int)std::lround(target_items / nog);
+    const double target_items = 4.0 * capacity[c];
+    int G = tunes[c].groups > 0 ? tunes[c].groups : (int)std::lround(target_items / nog);
     G = std::max(1, std::min<int>(G, (int)ct[c].size()));
     const int64_t row = sc.nObs * sc.nT * 2 * (int64_t)asz;
     // partial rows take at most a quarter of the case's element records, or 8 GiB if that is
@@ -609,7 +706,7 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     // [0, 1]; at four items per slot k = 3 and the first group is about 20 times the last).
     // The queues run large groups first, so the launch ends on small items and the workgroups
     // finish close together.  k depends on this case and the device alone.
-    const double k = size_decay((double)G * nog / capacity);
+    const double k = size_decay((double)G * nog / capacity[c]);
     auto cum = [k](double x) { return size_profile(k, x); };
     for (int g = 0; g < G; g++) {
       const double x = (double)(g + 1) / G;
@@ -637,10 +734,10 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
       t0 = t1;
     }
   }
-  // Still-air items first (one launch), then moving-medium items (another).  Within a launch the
-  // tile groups (all observer groups of one group of tiles) are spread over the kQueues queues,
-  // longest first onto the least loaded queue; a queue lists its groups longest first with each
-  // group's items adjacent, so the waves of one XCD work on the same records at the same time.
+  // Items grouped by launch.  Within a launch the tile groups (all observer groups of one group
+  // of tiles) are spread over the kQueues queues, longest first onto the least loaded queue; a
+  // queue lists its groups longest first with each group's items adjacent, so the waves of one
+  // XCD work on the same records at the same time.
   {
     struct Unit {
       int icase, group, queue;
@@ -658,10 +755,11 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
       }
       units[f->second].cost += it.cost;
     }
-    for (int kind = 0; kind < 2; kind++) {  // 0: still air, 1: moving
+    const int nl = (int)p.launches.size();
+    for (int l = 0; l < nl; l++) {
       std::vector<int> ids;
       for (int u = 0; u < (int)units.size(); u++)
-        if ((st.cases[units[u].icase].k.still ? 0 : 1) == kind) ids.push_back(u);
+        if (kind_of[units[u].icase] == l) ids.push_back(u);
       std::stable_sort(ids.begin(), ids.end(),
                        [&](int a, int b) { return units[a].cost > units[b].cost; });
       int64_t load[kQueues] = {};
@@ -676,29 +774,40 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
     std::stable_sort(p.items.begin(), p.items.end(), [&](const Item& a, const Item& b) {
       const Unit& ua = units[uid[std::make_pair(a.icase, a.group)]];
       const Unit& ub = units[uid[std::make_pair(b.icase, b.group)]];
-      const int sa = st.cases[a.icase].k.still, sb = st.cases[b.icase].k.still;
-      if (sa != sb) return sa > sb;
+      const int la = kind_of[a.icase], lb = kind_of[b.icase];
+      if (la != lb) return la < lb;
       if (ua.queue != ub.queue) return ua.queue < ub.queue;
       if (ua.cost != ub.cost) return ua.cost > ub.cost;
       if (a.icase != b.icase) return a.icase < b.icase;
       if (a.group != b.group) return a.group < b.group;
       return a.obs0 < b.obs0;
     });
-    p.n_still = 0;
-    for (const Item& it : p.items) p.n_still += st.cases[it.icase].k.still ? 1 : 0;
-    for (int l = 0; l < 2; l++) {
-      const int lo = l == 0 ? 0 : p.n_still, hi = l == 0 ? p.n_still : (int)p.items.size();
-      int i = lo;
+    int i = 0;
+    for (int l = 0; l < nl; l++) {
+      Launch& L = p.launches[l];
+      L.item0 = i;
+      int cap = 0;
       for (int q = 0; q < kQueues; q++) {
-        p.qoff[l][q] = i - lo;
-        while (i < hi && units[uid[std::make_pair(p.items[i].icase, p.items[i].group)]].queue == q) i++;
+        L.qoff[q] = i - L.item0;
+        while (i < (int)p.items.size() && kind_of[p.items[i].icase] == l &&
+               units[uid[std::make_pair(p.items[i].icase, p.items[i].group)]].queue == q)
+          i++;
       }
-      p.qoff[l][kQueues] = i - lo;
+      L.qoff[kQueues] = i - L.item0;
+      L.nitems = i - L.item0;
+      for (int c = 0; c < nc; c++)
+        if (kind_of[c] == l) cap = capacity[c];
+      L.grid = std::min(L.nitems, cap);
     }
   }
+  const int nl = (int)p.launches.size();
+  std::vector<int> qoff((size_t)nl * (kQueues + 1));
+  for (int l = 0; l < nl; l++)
+    for (int q = 0; q <= kQueues; q++) qoff[(size_t)l * (kQueues + 1) + q] = p.launches[l].qoff[q];
   HIPCHK(ctx, hipMalloc(&p.d_tiles, p.tiles.size() * sizeof(Tile)));
   HIPCHK(ctx, hipMalloc(&p.d_items, p.items.size() * sizeof(Item)));
-  HIPCHK(ctx, hipMalloc(&p.d_qoff, sizeof(p.qoff)));
+  HIPCHK(ctx, hipMalloc(&p.d_qoff, qoff.size() * sizeof(int)));
+  HIPCHK(ctx, hipMalloc(&p.d_counter, (size_t)nl * kQueues * sizeof(int)));
   HIPCHK(ctx, hipMalloc(&p.d_cases, nc * sizeof(TonalCaseDev)));
   HIPCHK(ctx, hipMalloc(&p.d_slab, p.slab_bytes));
   HIPCHK(ctx, hipMalloc(&p.d_flags, nc * sizeof(uint32_t)));
@@ -721,33 +830,32 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
   }
   HIPCHK(ctx, hipMemcpyAsync(p.d_tiles, p.tiles.data(), p.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(p.d_items, p.items.data(), p.items.size() * sizeof(Item), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(ctx, hipMemcpyAsync(p.d_qoff, p.qoff, sizeof(p.qoff), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(p.d_qoff, qoff.data(), qoff.size() * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipMemcpyAsync(p.d_cases, hc.data(), nc * sizeof(TonalCaseDev), hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-  p.grid[0] = std::min<int>(p.n_still, capacity);
-  p.grid[1] = std::min<int>((int)p.items.size() - p.n_still, capacity);
   p.valid = true;
   return ACUM_OK;
 }
 
-// Enqueues one full tonal pass: reset, zero partials, eval, reductions.
+// Enqueues one full tonal pass: clear queue heads, flags and partials, the eval launches, the
+// reductions.  Only kernels (no memset nodes), so the pass can be captured into a graph.
 int enqueue_pass(acum_ctx* ctx, Stage& st, int prec, bool with_events) {
   Plan& p = st.plan[prec];
   const bool mixed = prec == ACUM_MIXED;
   hipStream_t s = ctx->stream;
+  const int nl = (int)p.launches.size();
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[0], s));
-  HIPCHK(ctx, launch_reset(s, ctx->d_counter, ctx->d_flags));
+  HIPCHK(ctx, launch_fill_zero(s, p.d_counter, (int64_t)nl * kQueues * sizeof(int)));
   HIPCHK(ctx, launch_fill_zero(s, p.d_flags, (int64_t)st.cases.size() * sizeof(uint32_t)));
   HIPCHK(ctx, launch_fill_zero(s, p.d_slab, p.slab_bytes));
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[1], s));
-  const int nmov = (int)p.items.size() - p.n_still;
-  if (p.n_still > 0)
-    HIPCHK(ctx, launch_tonal_eval(mixed, false, p.tune.window, p.grid[0], s, p.d_cases, p.d_tiles,
-                                  p.d_items, p.d_qoff, ctx->d_counter));
-  if (nmov > 0)
-    HIPCHK(ctx, launch_tonal_eval(mixed, true, p.tune.window, p.grid[1], s, p.d_cases, p.d_tiles,
-                                  p.d_items + p.n_still, p.d_qoff + (kQueues + 1),
-                                  ctx->d_counter + kQueues));
+  for (int l = 0; l < nl; l++) {
+    const Launch& L = p.launches[l];
+    if (L.nitems > 0)
+      HIPCHK(ctx, launch_tonal_eval(mixed, L.moving, L.window, L.grid, s, p.d_cases, p.d_tiles,
+                                    p.d_items + L.item0, p.d_qoff + (size_t)l * (kQueues + 1),
+                                    p.d_counter + (size_t)l * kQueues));
+  }
   if (with_events) HIPCHK(ctx, hipEventRecord(ctx->ev[2], s));
   for (int c = 0; c < (int)st.cases.size(); c++)
     HIPCHK(ctx, launch_tonal_reduce(mixed, s, p.d_cases, c, (int)st.cases[c].nT, (int)st.cases[c].nObs));
@@ -863,7 +971,8 @@ int run_stage_once(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_gra
   t.n_partials = 0;
   for (int g : p.G) t.n_partials = std::max(t.n_partials, g);
   t.n_items = (int32_t)p.items.size();
-  t.launches = 3 + (int32_t)st.cases.size();
+  t.launches = 3 + (int32_t)p.launches.size() + (int32_t)st.cases.size();
+  t.eval_launches = (int32_t)p.launches.size();
   for (auto& sc : st.cases) t.motion_widened_cases += sc.motion_level > 0 ? 1 : 0;
   return check_flags(ctx, st, prec, retry);
 }
@@ -898,6 +1007,76 @@ acum_tune default_tune(int prec) {
   t.waves = 1;
   t.motion_permille = kMotionPermille;
   return t;
+}
+
+// Successive halving over tile shapes for one case, timed on a view of the case restricted to
+// its first kTuneObs observers (the view shares the case's device data; only its plans are its
+// own and freed here).  Round 0 times every window x spread x source-motion candidate (with the
+// context's tile length and other limits) for one pass; the best quarter goes on, joined by
+// each survivor with the other tile length, for two passes; the best quarter of those for four
+// passes, and the fastest of them wins.  Windows above 24 bins are left out: their LDS halves
+// the kernel's occupancy, which has never paid.
+constexpr int kTuneObs = 128;
+
+int tune_case(acum_ctx* ctx, const StagedCase& sc, int prec, TuneEntry* out, int* passes) {
+  const bool mixed = prec == ACUM_MIXED;
+  const acum_tune keep = ctx->tune[prec];
+  Stage view;
+  view.cases.push_back(sc);
+  StagedCase& v = view.cases[0];
+  v.nObs = std::min<int64_t>(sc.nObs, kTuneObs);
+  v.forced = true;
+  struct Cand {
+    acum_tune t;
+    double ms;
+  };
+  std::vector<Cand> cand;
+  for (int W : {16, 20, 24}) {
+    if (!eval_window_supported(mixed, W)) continue;
+    for (int sp : {2, 3, 4})
+      for (int pm : {600, 800}) {
+        acum_tune t = keep;
+        t.window = W;
+        t.max_spread = sp;
+        t.motion_permille = pm;
+        t.waves = 1;
+        cand.push_back(Cand{t, 0.0});
+      }
+  }
+  auto time_one = [&](Cand& c, int n) -> int {
+    v.force = c.t;
+    v.motion_level = 0;
+    for (auto& p : view.plan) free_plan(p);
+    int rc = run_stage(ctx, view, prec, 1, false);  // plan and warm
+    if (rc == ACUM_OK) rc = run_stage(ctx, view, prec, n, false);
+    c.ms = rc == ACUM_OK ? ctx->timing.total_ms : 1e300;
+    *passes += n;
+    return rc == ACUM_ERR_PLAN ? ACUM_OK : rc;  // a shape whose windows cannot hold the case loses
+  };
+  int rc = ACUM_OK;
+  for (int round = 0, n = 1; round < 3 && rc == ACUM_OK; round++, n *= 2) {
+    for (auto& c : cand)
+      if ((rc = time_one(c, n)) != ACUM_OK) break;
+    if (rc) break;
+    std::stable_sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.ms < b.ms; });
+    if (round == 2) break;
+    cand.resize(std::max<size_t>(1, (cand.size() + 3) / 4));
+    if (round == 0) {  // the other tile length for the survivors
+      const size_t m = cand.size();
+      for (size_t i = 0; i < m; i++) {
+        Cand c = cand[i];
+        c.t.max_tile = keep.max_tile > 64 ? 64 : 256;
+        cand.push_back(c);
+      }
+    }
+  }
+  for (auto& p : view.plan) free_plan(p);
+  if (rc) return rc;
+  if (cand.empty() || !(cand[0].ms < 1e300)) return ctx_fail(ctx, ACUM_ERR_PLAN, "no candidate tile shape ran");
+  out->tune = cand[0].t;
+  out->tune.waves = 1;
+  out->ms = cand[0].ms;
+  return ACUM_OK;
 }
 
 }  // namespace
@@ -1135,45 +1314,94 @@ int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* b
   Stage* st = get_stage(ctx, handle);
   if (!st) return ctx_fail(ctx, ACUM_ERR_ARG, "unknown handle");
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  const bool mixed = precision == ACUM_MIXED;
-  std::vector<acum_tune> cand;
-  // the window sets the occupancy and the spread the tile size; the tile length matters little
-  for (int W : {16, 20, 24, 32}) {
-    if (!eval_window_supported(mixed, W)) continue;
-    for (int sp : {2, 3, 4})
-      for (int mt : {256}) {
-        acum_tune t = default_tune(precision);
-        t.window = W;
-        t.max_spread = std::max(1, sp);
-        t.max_tile = mt;
-        cand.push_back(t);
-      }
+  for (auto& sc : st->cases) {
+    int rc = ensure_records(ctx, sc, precision);  // before any view copies the case
+    if (rc) return rc;
   }
-  const acum_tune keep = ctx->tune[precision];
-  double bms = 1e300;
-  acum_tune bt = keep;
-  auto fresh = [&]() {  // each shape starts from the reduced motion bound (its own fallback)
-    for (auto& sc : st->cases) sc.motion_level = 0;
-    for (auto& p : st->plan) free_plan(p);
-  };
-  for (const acum_tune& t : cand) {
-    ctx->tune[precision] = t;
-    fresh();
-    int rc = run_stage(ctx, *st, precision, 1, false);  // warm + plan
-    if (rc == ACUM_OK) rc = run_stage(ctx, *st, precision, 2, false);
-    if (rc == ACUM_OK && ctx->timing.total_ms < bms) {
-      bms = ctx->timing.total_ms;
-      bt = t;
+  int tuned = 0, hits = 0, passes = 0;
+  for (auto& sc : st->cases) {
+    const TuneKey key = tune_key(ctx, sc, precision);
+    if (ctx->tune_cache.count(key)) {
+      hits++;
+      continue;
     }
+    TuneEntry e{};
+    int rc = tune_case(ctx, sc, precision, &e, &passes);
+    if (rc) return rc;
+    ctx->tune_cache[key] = e;
+    tuned++;
   }
-  if (bms == 1e300) {
-    ctx->tune[precision] = keep;
-    return ctx_fail(ctx, ACUM_ERR_PLAN, "no candidate tile shape ran");
+  for (auto& p : st->plan) free_plan(p);  // the next run plans with the cached shapes
+  const auto it = ctx->tune_cache.find(tune_key(ctx, st->cases[0], precision));
+  if (best) *best = it->second.tune;
+  if (best_ms) *best_ms = it->second.ms;
+  ctx->timing = acum_timing();
+  ctx->timing.tune_cases = tuned;
+  ctx->timing.tune_hits = hits;
+  ctx->timing.tune_passes = passes;
+  return ACUM_OK;
+}
+
+int acum_get_case_tuning(acum_ctx* ctx, int32_t handle, int32_t icase, int32_t precision, acum_tune* t) {
+  if (!ctx || !t || (precision != ACUM_FP64 && precision != ACUM_MIXED)) return ACUM_ERR_ARG;
+  Stage* st = get_stage(ctx, handle);
+  if (!st || icase < 0 || icase >= (int)st->cases.size())
+    return ctx_fail(ctx, ACUM_ERR_ARG, "unknown handle or case");
+  *t = case_tune(ctx, st->cases[icase], precision);
+  return ACUM_OK;
+}
+
+int acum_tune_cache_save(const acum_ctx* ctx, const char* path) {
+  if (!ctx || !path) return ACUM_ERR_ARG;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return ACUM_ERR_IO;
+  std::fprintf(f, "# acum tune cache: prec moving lgE lgT lgO cus | window spread tile groups chunk motion | ms\n");
+  for (const auto& kv : ctx->tune_cache) {
+    const TuneKey& k = kv.first;
+    const acum_tune& t = kv.second.tune;
+    std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d %d %d %.6f\n", k.prec, k.moving, k.lgE, k.lgT, k.lgO, k.cus,
+                 t.window, t.max_spread, t.max_tile, t.groups, t.max_chunk, t.motion_permille, kv.second.ms);
   }
-  ctx->tune[precision] = bt;
-  fresh();
-  if (best) *best = bt;
-  if (best_ms) *best_ms = bms;
+  return std::fclose(f) == 0 ? ACUM_OK : ACUM_ERR_IO;
+}
+
+int acum_tune_cache_load(acum_ctx* ctx, const char* path) {
+  if (!ctx || !path) return ACUM_ERR_ARG;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return ctx_fail(ctx, ACUM_ERR_IO, std::string("cannot open tune cache ") + path);
+  char line[512];
+  std::map<TuneKey, TuneEntry> add;
+  int rc = ACUM_OK;
+  while (std::fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    TuneKey k{};
+    acum_tune t{};
+    double ms = 0;
+    if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d %d %lf", &k.prec, &k.moving, &k.lgE, &k.lgT, &k.lgO,
+                    &k.cus, &t.window, &t.max_spread, &t.max_tile, &t.groups, &t.max_chunk,
+                    &t.motion_permille, &ms) != 13 ||
+        (k.prec != ACUM_FP64 && k.prec != ACUM_MIXED) || !eval_window_supported(k.prec == ACUM_MIXED, t.window) ||
+        t.max_spread < 1 || t.max_tile < 1 || t.max_chunk < 1 || t.groups < 0 ||
+        (t.motion_permille != 0 && (t.motion_permille < 100 || t.motion_permille > 1000))) {
+      rc = ctx_fail(ctx, ACUM_ERR_IO, std::string("malformed tune cache line: ") + line);
+      break;
+    }
+    t.waves = 1;
+    add[k] = TuneEntry{t, ms};
+  }
+  std::fclose(f);
+  if (rc) return rc;
+  for (auto& kv : add) ctx->tune_cache[kv.first] = kv.second;
+  for (auto& kv : ctx->stages)  // plans made before see the new shapes on their next run
+    for (auto& p : kv.second->plan) free_plan(p);
+  return ACUM_OK;
+}
+
+int acum_tune_cache_clear(acum_ctx* ctx) {
+  if (!ctx) return ACUM_ERR_ARG;
+  ctx->tune_cache.clear();
+  for (auto& kv : ctx->stages)
+    for (auto& p : kv.second->plan) free_plan(p);
   return ACUM_OK;
 }
 

@@ -108,6 +108,12 @@ typedef struct acum_timing {
   int32_t launches;      /* kernel launches of the run                                       */
   int32_t motion_widened_cases; /* cases planned with a wider source-motion bound than tuned */
                                 /* after a window overflow (see acum_tune)                    */
+  int32_t eval_launches; /* eval kernel launches of a pass: one per kernel variant in use     */
+                         /* (still air or moving medium, window size)                         */
+  /* after acum_autotune: */
+  int32_t tune_cases;    /* cases whose bucket was tuned by this call                          */
+  int32_t tune_hits;     /* cases whose bucket was already in the context's cache             */
+  int32_t tune_passes;   /* timed passes the tuning took                                       */
 } acum_timing;
 
 /* Tile shapes of the tonal kernel (chosen by acum_autotune or set by the caller). */
@@ -153,10 +159,28 @@ int acum_tonal_release(acum_ctx* ctx, int32_t handle);
 int acum_tonal_bins(acum_ctx* ctx, const acum_tonal_case* c, int64_t iobs,
                     int32_t* ipts, int32_t* npi, int32_t* ndown, int32_t* nup);
 int acum_get_timing(const acum_ctx* ctx, acum_timing* t);
+/* The context's tile shape per precision: what a case plans with unless the autotuner has a
+ * shape for its bucket.  Replaces the compile-time launch tiles of main.cu:194-200.            */
 int acum_get_tuning(const acum_ctx* ctx, int32_t precision, acum_tune* t);
 int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t);
-/* Times candidate tile shapes on the staged case and keeps the fastest (per precision).      */
+/* Autotuner.  Each case of the staged batch falls in a shape bucket: precision, still air or
+ * moving medium, the rounded-up base-2 logarithms of its element, step and observer counts, and
+ * the device's compute units.  A bucket not yet in the context's cache is tuned on its case:
+ * successive halving over window x spread x source-motion bound, then max_tile, each candidate
+ * timed on the case's first 128 observers, keeping the best third per round with twice the
+ * passes.  The chosen shape is cached by bucket, and every later plan of a case in that bucket
+ * (alone or in any batch) uses it, so batched and single runs of a case stay bit-identical.
+ * best / best_ms: the shape and time per pass (on the observer subset) of case 0's bucket.
+ * acum_get_timing then reports tune_cases, tune_hits and tune_passes of the call.             */
 int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* best, double* best_ms);
+/* The shape case icase of a staged batch plans with for a precision (from the cache or the
+ * context's shape).                                                                          */
+int acum_get_case_tuning(acum_ctx* ctx, int32_t handle, int32_t icase, int32_t precision, acum_tune* t);
+/* The autotuner's cache as a text file (one bucket per line), to keep tuned shapes across
+ * processes; loading adds to (and overrides within) the context's cache; clear empties it.   */
+int acum_tune_cache_save(const acum_ctx* ctx, const char* path);
+int acum_tune_cache_load(acum_ctx* ctx, const char* path);
+int acum_tune_cache_clear(acum_ctx* ctx);
 
 /* --- broadband --------------------------------------------------------------------------- */
 /* Replaces main.cu:150 (findBBnoise, bb_core.cu:11).  BBspl[nObs*nf2*7], BBoaspl[nObs].      */
