@@ -1,26 +1,40 @@
 #!/usr/bin/env python3
-"""Throughput of the tonal Farassat 1A hot path on one MI355X.
+"""Throughput of the tonal Farassat 1A hot path on one MI355X, with the broadband, batched and stress
+workloads beside it.
 
-Workload: the four-blade rotor in forward flight -- 4 blades x 20,000 panels (100 chordwise x 100
-spanwise per side), 1024 source steps per revolution, 1024 observers on a 20 m hemisphere,
-oM = 250 rad/s, Minf = 0.1 at -5 deg: 8.39e10 source-observer-time evaluations per pass.
-Synthetic, seeded input in the reference's layout, preprocessed on the host before timing; the
-element data is resident in HBM when the timed region starts.
+Headline workload: the four-blade rotor in forward flight -- 4 blades x 20,000 panels (100
+chordwise x 100 spanwise per side), 1024 source steps per revolution, 1024 observers on a 20 m
+hemisphere, oM = 250 rad/s, Minf = 0.1 at -5 deg: 8.39e10 source-observer-time evaluations per
+pass.  Synthetic, seeded input in the reference's layout, preprocessed on the host before timing;
+the element data is resident in HBM when the timed region starts.
 
 A step is one full pass of the mixed FP32/FP64 path: clear partials, eval + binning kernel,
-ordered Kahan reduction to pT/pL/pA.  The plain FP64 path is timed the same way and reported
-beside it, and the broadband (BPM) path is timed on its own workload (4 x 60 sections, 360
-steps, 1024 observers, 34 bands).  Accuracy of the timed outputs is checked against the FP64 CPU
-oracle on the first observers (that oracle run is also the CPU baseline).
+ordered Kahan reduction to pT/pL/pA.  Reported beside it, each in its own JSON key:
+  fp64_path  the plain FP64 path on the same case;
+  accuracy   both paths against the FP64 CPU oracle on 32 observers spread over the hemisphere:
+             rel-L2, OASPL and 1/3-octave band levels (that oracle run, median of three, is also
+             the CPU baseline; the float instance of the same restatement is timed beside it);
+  bpm        the broadband path (4 x 60 sections, 360 steps, 1024 observers, 34 bands), against
+             the FP64 oracle, with the CPU restatement on all threads and on one;
+  batched    the first 8 of the 64 seeded uneven cases, 4 per persistent launch, tile shapes per
+             case from the autotuner, HIP graph replay, the FP64 path on the same launches, every
+             case equal to it run alone and checked against the oracle;
+  stress     the coaxial 2 x 4-blade case (3.36e12 evals), against the oracle on 16 observers.
+The bench exits with status 3 (after printing its line) when a parity bar is broken: mixed-path
+OASPL or a band level more than 0.01 dB from the oracle, FP64 rel-L2 above 1e-6, broadband levels
+more than 0.01 dB off.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--oracle-obs N] [--no-fp64] [--no-bpm]
+                    [--no-batched] [--no-stress] [--cases N --wave M]
+    python bench.py --workload batched|stress|bpm          # one leg alone (batched: --cases of 64)
     python bench.py --summarize-pmc FETCH_DIR WRITE_DIR   # HBM traffic summary into profiles/
     python bench.py --summarize-sq DIR                     # instruction mix into profiles/
     python bench.py --summarize-stats DIR                  # kernel times into profiles/
 
 Roofline of the eval kernel: 100 FLOP per eval (70 FP32 + 30 FP64 in the mixed path) against
-the MI355X vector peaks; `traffic` is the kernel's measured HBM bytes per launch, read from the
-committed rocprofv3 counter summary (profiles/tonal_eval_c2_hbm_traffic.md) when present.
+the MI355X vector peaks (it is bound by vector-ALU issue); `traffic` is the kernel's measured HBM
+bytes per launch, read from the committed rocprofv3 counter summary
+(profiles/tonal_eval_c2_hbm_traffic.md) when present.
 """
 import argparse
 import ctypes as C
@@ -28,6 +42,7 @@ import glob
 import json
 import os
 import re
+import subprocess
 import sys
 import time
 
@@ -212,9 +227,63 @@ def summarize_stats(d, workload="rotor"):
     return 0
 
 
-def bench_bpm(acum, O, ctx, steps):
-    """The broadband path on the c3 workload: GPU kernels and the whole call, and the
-    reference's own single-threaded findBBnoise (oracle/_ref) on a bounded observer sample."""
+def note(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_info():
+    """CPU model (lscpu), CPUs visible to this process, and the threads the CPU baselines use: all
+    of the process's CPUs, or OMP_NUM_THREADS when the host sets a smaller share."""
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None and os.path.exists("/proc/cpuinfo"):
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    cpus = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(cpus, omp) if omp > 0 else cpus
+    return dict(cpu_model=model, cpus_visible=cpus, nproc=os.cpu_count(), omp_num_threads=omp or None,
+                threads=threads)
+
+
+def timed_runs(fn, runs):
+    """(first result, median seconds, all seconds) of `runs` calls of fn."""
+    first, secs = None, []
+    for i in range(runs):
+        t = time.perf_counter()
+        r = fn()
+        secs.append(time.perf_counter() - t)
+        if i == 0:
+            first = r
+    return first, float(np.median(secs)), secs
+
+
+def tonal_accuracy(acum, O, P, g, o, dt):
+    """Mixed-path histories g (observer columns of the oracle subset) against the FP64 oracle's o:
+    rel-L2 per channel, OASPL and 1/3-octave band levels (bands above the -100 dB floor)."""
+    ref_spl, _ = P.band_levels(o[2], dt)
+    spl, _ = P.band_levels(g[2], dt)
+    d_band, where = P.band_error(spl, ref_spl)
+    return dict(observers=int(g[2].shape[1]), rel_l2_pA=O.rel_l2(g[2], o[2]), rel_l2_pT=O.rel_l2(g[0], o[0]),
+                rel_l2_pL=O.rel_l2(g[1], o[1]),
+                max_doaspl_db=float(np.abs(acum.oaspl(g[2]) - acum.oaspl(o[2].astype(np.float32))).max()),
+                max_dband_db=d_band,
+                worst_band_hz=None if where is None else float(P.ISO_BANDS[where[1]]),
+                live_bands=int((ref_spl > -99.0).sum()))
+
+
+def leg_bpm(acum, O, ctx, steps, host):
+    """The broadband path on the 4 x 60-section workload: GPU kernels and the whole call, accuracy
+    against the FP64 oracle on 8 observers, and the CPU restatement timed on all threads and on one
+    (the reference's own loop is single-threaded, bb_core.cu:449)."""
     run = acum.Run.synth(acum.synth_params(3))
     c = run.bpm_case()
     ctx.bpm(c)  # warm
@@ -239,110 +308,130 @@ def bench_bpm(acum, O, ctx, steps):
                call_ms=round(float(np.median(call_ms)), 2),
                value=round(evals / (g * 1e-3) / 1e9, 3), unit="G src-obs-time evals/s",
                band_evals_per_s=round(band_evals / (g * 1e-3) / 1e9, 2),
-               roofline={"bound": "mfma", "engines": "VALU (FP32, transcendental)",
+               roofline={"bound": "valu", "engines": "VALU (FP32, transcendental)",
                          "achieved": round(flop / (g * 1e-3) / 1e12, 3),
                          "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                          "frac": round(flop / (g * 1e-3) / PEAK_FP32, 4),
                          "flop": flop, "kernels": "bpm_shape + bpm_obs + bpm_final"})
-    # accuracy on the first observers against the FP64 oracle
-    nob = 8
-    ospl, ooa = O.bpm_f64(c, 0, nob)
-    out["max_dspl_db_vs_oracle"] = float(np.abs(spl[:nob].astype(np.float64) - ospl).max())
-    out["max_doaspl_db_vs_oracle"] = float(np.abs(oa[:nob] - ooa).max())
-    if O.ref_available():
-        nref = 2
-        devnull = os.open(os.devnull, os.O_WRONLY)
-        saved = os.dup(1)
-        sys.stdout.flush()
-        os.dup2(devnull, 1)  # the reference prints progress lines
-        try:
-            t = time.perf_counter()
-            O.ref_bpm(c, 0, nref)
-            dt = time.perf_counter() - t
-        finally:
-            os.dup2(saved, 1)
-            os.close(devnull)
-            os.close(saved)
-        out["cpu_baseline"] = dict(value=round(nsrc * nref / dt / 1e9, 6), unit="G src-obs-time evals/s",
-                                   cores=1, kind="reference",
-                                   sample=f"findBBnoise built from the reference sources, first {nref} "
-                                          f"observers ({nsrc * nref:.3g} evals, {dt:.1f} s)")
-        out["vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
+    worst = worst_oa = 0.0
+    for obs0 in (0, c.nObs - 4):
+        ospl, ooa = O.bpm_f64(c, obs0, 4, host["threads"])
+        worst = max(worst, float(np.abs(spl[obs0:obs0 + 4].astype(np.float64) - ospl).max()))
+        worst_oa = max(worst_oa, float(np.abs(oa[obs0:obs0 + 4] - ooa).max()))
+    out["accuracy"] = dict(observers=8, max_dspl_db=worst, max_doaspl_db=worst_oa)
+    # CPU baselines: the FP64 restatement (orc_bpm), median of three runs each
+    nall, n1 = 16, 2
+    _, s_all, _ = timed_runs(lambda: O.bpm_f64(c, 0, nall, host["threads"]), 3)
+    _, s_one, _ = timed_runs(lambda: O.bpm_f64(c, 0, n1, 1), 3)
+    out["cpu_baseline"] = dict(value=round(nsrc * nall / s_all / 1e9, 6), unit="G src-obs-time evals/s",
+                               cores=host["threads"], kind="port", runs=3,
+                               sample=f"FP64 restatement of findBBnoise, first {nall} observers "
+                                      f"({nsrc * nall:.3g} evals, median {s_all:.2f} s)")
+    out["cpu_baseline_1thread"] = dict(value=round(nsrc * n1 / s_one / 1e9, 6), unit="G src-obs-time evals/s",
+                                       cores=1, kind="port", runs=3,
+                                       sample=f"same, first {n1} observers ({nsrc * n1:.3g} evals, "
+                                              f"median {s_one:.2f} s)")
+    out["vs_cpu"] = round(out["value"] / out["cpu_baseline"]["value"], 1)
     return out
 
 
-def note(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def bench_batched(acum, O, args):
-    """The batched workload: 64 seeded rotor cases of uneven size (2-6 blades, 2,000-40,000
+def leg_batched(acum, O, P, ctx, host, ncases, per_launch, oracle_obs, steps=2, fp64=True):
+    """The batched workload's first `ncases` seeded rotor cases (of 64: 2-6 blades, 2,000-40,000
     elements per blade, 256-2048 steps, 16-2048 observers on spheres of 5R-50R, oM 150-400,
-    Minf 0-0.2), run as batches of `--wave` cases per persistent launch (all 64 at once exceed
-    the host and device memory), each batch replayed as a captured HIP graph.  The tile shape
-    comes from the autotuner on the first case; one case per batch is also run alone and must
-    give bit-identical histories."""
-    ctx = acum.Context(0)
+    Minf 0-0.2), `per_launch` cases per persistent launch: tile shapes per case from the
+    autotuner, the timed passes replayed as a captured HIP graph, the FP64 path on the same
+    launches, every case checked bit for bit against it run alone and against the FP64 oracle on
+    its first observers."""
     seed = 31004
-    total_evals = 0
-    gpu_ms = 0.0
-    host_s = 0.0
-    tuned = None
-    checked = 0
-    widened = 0  # cases that needed a wider source-motion bound than the default
-    for w0 in range(0, args.cases, args.wave):
-        idx = list(range(w0, min(args.cases, w0 + args.wave)))
+    tot_evals = 0
+    mixed_ms = fp64_ms = eval_ms = 0.0
+    host_s = tune_s = 0.0
+    tuned = hits = widened = launches = 0
+    acc = []
+    windows = {}
+    cpu_evals, cpu_s, cpu_obs = 0, 0.0, 0
+    for w0 in range(0, ncases, per_launch):
+        idx = list(range(w0, min(ncases, w0 + per_launch)))
         t0 = time.time()
         runs = [acum.Run.synth(acum.batch_params(seed, i)).preprocess() for i in idx]
         cases = [r.tonal_case(acum.MIXED) for r in runs]
         host_s += time.time() - t0
-        if tuned is None:
-            h1 = ctx.stage([cases[0]])
-            tuned, tms = ctx.autotune(h1, acum.MIXED)
-            ctx.release(h1)
-            note(f"autotuned tile shape: window {tuned.window}, spread {tuned.max_spread}, "
-                 f"tile {tuned.max_tile} ({tms:.1f} ms on case 0)")
         h = ctx.stage(cases)
-        ctx.run(h, acum.MIXED, repeats=1, graph=True)  # plan, capture, warm
-        tm = ctx.run(h, acum.MIXED, repeats=2, graph=True)
-        gpu_ms += tm.total_ms
+        t0 = time.time()
+        ctx.autotune(h, acum.MIXED)
+        tune_s += time.time() - t0
+        tt = ctx.timing()
+        tuned += tt.tune_cases
+        hits += tt.tune_hits
+        for i in range(len(cases)):
+            w = ctx.case_tuning(h, i, acum.MIXED).window
+            windows[w] = windows.get(w, 0) + 1
+        tm = ctx.run(h, acum.MIXED)  # plan and warm, with per-launch kernel times
+        eval_ms += tm.eval_ms
+        launches = max(launches, tm.eval_launches)
+        ctx.run(h, acum.MIXED, repeats=1, graph=True)  # capture
+        tm = ctx.run(h, acum.MIXED, repeats=steps, graph=True)
+        mixed_ms += tm.total_ms
         widened += tm.motion_widened_cases
         ev = sum(sum(c.E[s] for s in range(c.nSurf)) * c.nTime * c.nObs for c in cases)
-        total_evals += ev
-        batch_out = ctx.fetch(h, 0, cases[0].nTime, cases[0].nObs)
+        tot_evals += ev
+        batch_out = [ctx.fetch(h, i, c.nTime, c.nObs) for i, c in enumerate(cases)]
+        if fp64:
+            ctx.run(h, acum.FP64)
+            fp64_ms += ctx.run(h, acum.FP64).total_ms
         ctx.release(h)
-        alone = ctx.tonal(cases[0])
-        same = all(np.array_equal(a, b) for a, b in zip(batch_out, alone))
-        if not same:
-            raise SystemExit(f"case {idx[0]}: batched result differs from the case run alone")
-        checked += 1
-        note(f"cases {idx[0]}-{idx[-1]}: {ev:.3g} evals, {tm.total_ms:.1f} ms per pass")
-        del runs, cases
-    value = total_evals / (gpu_ms * 1e-3) / 1e9
-    out = {"metric": "G src-obs-time evals/s (batched, mixed)", "value": round(value, 3),
-           "unit": "G src-obs-time evals/s", "n_gpus": 1, "higher_is_better": True,
-           "dtype": "f32+f64", "data": "synthetic",
-           "config": {"workload": f"{args.cases} seeded rotor cases of uneven size in batches of "
-                                  f"{args.wave}, HIP graph replay", "evals": total_evals,
-                      "tile": {"window": tuned.window, "max_spread": tuned.max_spread,
-                               "max_tile": tuned.max_tile}},
-           "gpu_s": round(gpu_ms / 1e3, 3), "host_synth_prep_s": round(host_s, 1),
-           "batches_checked_against_alone": checked,
-           "cases_on_wider_motion_bound": widened}
-    ctx.close()
-    print(json.dumps(out))
-    return 0
+        for i, (c, b) in enumerate(zip(cases, batch_out)):
+            alone = ctx.tonal(c)
+            if not all(np.array_equal(x, y) for x, y in zip(alone, b)):
+                raise SystemExit(f"batched case {idx[i]}: histories differ from the case run alone")
+            sub = P.obs_subset(c, np.arange(min(oracle_obs, c.nObs)), acum.FP64)
+            o = O.tonal_f64(sub, host["threads"])
+            acc.append(tonal_accuracy(acum, O, P, [x[:, :sub.nObs] for x in b], o, c.dPsi / c.oM))
+            if w0 == 0 and i == 0:
+                # CPU baseline: the oracle is parallel over observers, so it is timed on as many
+                # observers of the first case as it has threads
+                cb = P.obs_subset(c, np.arange(min(host["threads"], c.nObs)), acum.FP64)
+                _, cpu_s, _ = timed_runs(lambda: O.tonal_f64(cb, host["threads"]), 3)
+                cpu_evals = sum(c.E[s] for s in range(c.nSurf)) * c.nTime * cb.nObs
+                cpu_obs = cb.nObs
+        note(f"batched cases {idx[0]}-{idx[-1]}: {ev:.3g} evals, {tm.total_ms:.1f} ms per mixed pass")
+        del runs, cases, batch_out
+    value = tot_evals / (mixed_ms * 1e-3) / 1e9
+    peak = mixed_peak()
+    out = dict(workload=f"first {ncases} of the 64 seeded uneven rotor cases, {per_launch} per persistent "
+                        f"launch, tile shapes per case from the autotuner, HIP graph replay",
+               evals=tot_evals, value=round(value, 3), unit="G src-obs-time evals/s",
+               gpu_ms_per_pass=round(mixed_ms, 2),
+               roofline={"bound": "valu", "kernel": "tonal_eval_kernel (all launches of a pass)",
+                         "achieved": round(FLOP_PER_EVAL * tot_evals / (eval_ms * 1e-3) / 1e12, 3),
+                         "peak": round(peak / 1e12, 3), "unit": "TFLOP/s",
+                         "frac": round(FLOP_PER_EVAL * tot_evals / (eval_ms * 1e-3) / peak, 4)},
+               cases_checked_against_alone=ncases,
+               accuracy=dict(observers_per_case=oracle_obs,
+                             max_rel_l2_pA=max(a["rel_l2_pA"] for a in acc),
+                             max_doaspl_db=max(a["max_doaspl_db"] for a in acc),
+                             max_dband_db=max(a["max_dband_db"] for a in acc)),
+               autotune=dict(buckets_tuned=tuned, cache_hits=hits, seconds=round(tune_s, 1),
+                             windows=windows),
+               eval_launches_per_pass=launches, cases_on_wider_motion_bound=widened,
+               host_synth_prep_s=round(host_s, 1),
+               cpu_baseline=dict(value=round(cpu_evals / cpu_s / 1e9, 4), unit="G src-obs-time evals/s",
+                                 cores=host["threads"], kind="port", runs=3,
+                                 sample=f"FP64 oracle, first {cpu_obs} observers of case 0 "
+                                        f"({cpu_evals:.3g} evals, median of 3 runs {cpu_s:.1f} s)"))
+    if fp64:
+        out["fp64_value"] = round(tot_evals / (fp64_ms * 1e-3) / 1e9, 3)
+        out["mixed_over_fp64"] = round(value / out["fp64_value"], 3)
+    return out
 
 
-def bench_stress(acum, O, args):
+def leg_stress(acum, O, P, ctx, host, steps, oracle_obs):
     """The stress workload: coaxial counter-rotating 2 x 4 blades, 50,000 elements per blade,
     2048 steps, 4096 observers on a 30R sphere, Minf 0.15, impulsive loading (3.36e12 evals),
-    mixed path, accuracy against the FP64 oracle on the first observers."""
+    mixed path, accuracy against the FP64 oracle on spread observers."""
     t0 = time.time()
-    note("synthesising the coaxial stress case")
     run = acum.Run.synth(acum.synth_params(5))
     synth_s = time.time() - t0
-    ctx = acum.Context(0)
     t0 = time.time()
     run.preprocess_gpu(ctx)
     prep_s = time.time() - t0
@@ -352,44 +441,56 @@ def bench_stress(acum, O, args):
     t0 = time.time()
     h = ctx.stage([case])
     stage_s = time.time() - t0
-    note("timing the mixed path")
     ctx.run(h, acum.MIXED)
-    ms = []
-    for _ in range(max(1, args.steps)):
-        ms.append(ctx.run(h, acum.MIXED).total_ms)
-    widened = ctx.timing().motion_widened_cases
+    ms, ems = [], []
+    for _ in range(max(1, steps)):
+        tm = ctx.run(h, acum.MIXED)
+        ms.append(tm.total_ms)
+        ems.append(tm.eval_ms)
+    widened = tm.motion_widened_cases
     g = ctx.fetch(h, 0, case.nTime, case.nObs)
     ctx.release(h)
-    nob = max(1, min(args.oracle_obs, case.nObs))
-    sub = acum.TonalCase()
-    C.pointer(sub)[0] = case
-    sub.nObs = nob
-    sub.precision = acum.FP64
-    note(f"CPU oracle on {nob} observers")
-    threads = min(len(os.sched_getaffinity(0)), 16, nob)  # the oracle is parallel over observers
+    idx = P.spread_observers(case.nObs, oracle_obs)
+    sub = P.obs_subset(case, idx, acum.FP64)
     t = time.perf_counter()
-    oT, oL, oA = O.tonal_f64(sub, threads)
+    o = O.tonal_f64(sub, host["threads"])
     cpu_s = time.perf_counter() - t
-    gA = g[2][:, :nob]
-    d = np.abs(acum.oaspl(gA) - acum.oaspl(oA.astype(np.float32)))
-    step_ms = float(np.median(ms))
-    out = {"metric": "G src-obs-time evals/s (stress, mixed)",
-           "value": round(evals / (step_ms * 1e-3) / 1e9, 3), "unit": "G src-obs-time evals/s",
-           "n_gpus": 1, "higher_is_better": True, "dtype": "f32+f64", "data": "synthetic",
-           "config": {"workload": "coaxial 2 x 4 blades x 50,000 elements, 2048 steps, 4096 "
-                                  "observers, Minf 0.15, impulsive loading", "evals": evals},
-           "ms_per_pass": round(step_ms, 1), "on_wider_motion_bound": bool(widened),
-           "accuracy": {"observers": nob, "rel_l2_pA": O.rel_l2(gA, oA),
-                        "rel_l2_pT": O.rel_l2(g[0][:, :nob], oT), "rel_l2_pL": O.rel_l2(g[1][:, :nob], oL),
-                        "max_doaspl_db": float(d.max())},
-           "cpu_baseline": {"value": round(E * case.nTime * nob / cpu_s / 1e9, 4),
-                            "unit": "G src-obs-time evals/s", "cores": threads, "kind": "port",
-                            "sample": f"FP64 oracle, first {nob} observers ({cpu_s:.1f} s)"},
-           "synth_s": round(synth_s, 1), "gpu_preprocess_call_s": round(prep_s, 1),
-           "stage_s": round(stage_s, 1)}
-    ctx.close()
-    print(json.dumps(out))
-    return 0
+    step_ms, eval_ms = float(np.median(ms)), float(np.median(ems))
+    peak = mixed_peak()
+    return dict(workload="coaxial 2 x 4 blades x 50,000 elements, 2048 steps, 4096 observers, Minf 0.15, "
+                         "impulsive loading", evals=evals,
+                value=round(evals / (step_ms * 1e-3) / 1e9, 3), unit="G src-obs-time evals/s",
+                ms_per_pass=round(step_ms, 1),
+                roofline={"bound": "valu", "kernel": "tonal_eval_kernel<mixed, moving>",
+                          "achieved": round(FLOP_PER_EVAL * evals / (eval_ms * 1e-3) / 1e12, 3),
+                          "peak": round(peak / 1e12, 3), "unit": "TFLOP/s",
+                          "frac": round(FLOP_PER_EVAL * evals / (eval_ms * 1e-3) / peak, 4)},
+                on_wider_motion_bound=bool(widened),
+                accuracy=tonal_accuracy(acum, O, P, [x[:, idx] for x in g], o, case.dPsi / case.oM),
+                cpu_baseline=dict(value=round(E * case.nTime * len(idx) / cpu_s / 1e9, 4),
+                                  unit="G src-obs-time evals/s", cores=host["threads"], kind="port", runs=1,
+                                  sample=f"FP64 oracle, {len(idx)} spread observers ({cpu_s:.1f} s)"),
+                synth_s=round(synth_s, 1), gpu_preprocess_call_s=round(prep_s, 1), stage_s=round(stage_s, 1))
+
+
+def parity_failures(out):
+    """The path's bars that a measured line breaks (the bench then exits with status 3)."""
+    bad = []
+    acc = out.get("accuracy") or {}
+    if acc.get("mixed_vs_oracle_max_doaspl_db", 0) > 0.01:
+        bad.append("rotor mixed OASPL")
+    if acc.get("mixed_vs_oracle_max_dband_db", 0) > 0.01:
+        bad.append("rotor mixed 1/3-octave band")
+    if acc.get("fp64_vs_oracle_rel_l2_pA", 0) > 1e-6:
+        bad.append("rotor FP64 rel-L2")
+    for leg in ("batched", "stress"):
+        a = (out.get(leg) or {}).get("accuracy") or {}
+        if a.get("max_doaspl_db", 0) > 0.01 or a.get("max_dband_db", 0) > 0.01:
+            bad.append(f"{leg} levels")
+    b = (out.get("bpm") or {}).get("accuracy") or {}
+    if b.get("max_dspl_db", 0) > 0.01 or b.get("max_doaspl_db", 0) > 0.01:
+        bad.append("broadband levels")
+    return bad
 
 
 def main():
@@ -397,9 +498,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--oracle-obs", type=int, default=64)
+    ap.add_argument("--oracle-obs", type=int, default=32,
+                    help="observers (spread over the layout) of the rotor's CPU baseline and accuracy")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-bpm", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-stress", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay the pass as a captured HIP graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy first (for scaling HBM counters)")
@@ -407,11 +511,11 @@ def main():
     ap.add_argument("--summarize-sq", metavar="DIR")
     ap.add_argument("--summarize-stats", metavar="DIR")
     ap.add_argument("--workload", choices=("rotor", "batched", "stress", "bpm"), default="rotor",
-                    help="rotor: the four-blade rotor (default, the headline line); batched: 64 "
-                         "uneven cases; stress: the coaxial 3.4e12-eval case; bpm: the "
-                         "broadband leg alone")
-    ap.add_argument("--cases", type=int, default=64)
-    ap.add_argument("--wave", type=int, default=8)
+                    help="rotor: the four-blade rotor line with the broadband, batched and stress legs "
+                         "(default); batched: the uneven cases alone; stress: the coaxial case alone; "
+                         "bpm: the broadband leg alone")
+    ap.add_argument("--cases", type=int, default=8, help="batched cases (of 64)")
+    ap.add_argument("--wave", type=int, default=4, help="batched cases per persistent launch")
     args = ap.parse_args()
     if args.summarize_stats:
         return summarize_stats(args.summarize_stats, args.workload)
@@ -427,18 +531,26 @@ def main():
     import acum3_periodic_amd as acum
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O  # tests/oracle.py: the oracle binding (CPU baseline and accuracy only)
+    import parity as P  # tests/parity.py: observer subsets and band-level comparison
 
+    host = host_info()
     if args.copy_calibration:
         copy_calibration()
-    if args.workload == "batched":
-        return bench_batched(acum, O, args)
-    if args.workload == "stress":
-        return bench_stress(acum, O, args)
-    if args.workload == "bpm":
+    if args.workload in ("batched", "stress", "bpm"):
         ctx = acum.Context(0)
-        out = bench_bpm(acum, O, ctx, args.steps)
+        if args.workload == "batched":
+            out = leg_batched(acum, O, P, ctx, host, args.cases, args.wave, 4, max(1, args.steps), not args.no_fp64)
+        elif args.workload == "stress":
+            out = leg_stress(acum, O, P, ctx, host, args.steps, args.oracle_obs)
+        else:
+            out = leg_bpm(acum, O, ctx, args.steps, host)
+        out["host"] = host
         ctx.close()
         print(json.dumps(out))
+        bad = parity_failures({args.workload: out})
+        if bad:
+            print("parity bars broken: " + ", ".join(bad), file=sys.stderr)
+            return 3
         return 0
 
     t0 = time.time()
@@ -466,8 +578,7 @@ def main():
         b = acum.case_arrays(run2.tonal_case(acum.MIXED))
         same = all(np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)) for k in ("X", "N", "V", "PR"))
         nrec = sum(E) * case.nTime
-        prep = dict(records=nrec, host_fp64_s=round(host_prep_s, 3),
-                    host_threads=min(len(os.sched_getaffinity(0)), 16),
+        prep = dict(records=nrec, host_fp64_s=round(host_prep_s, 3), host_threads=host["threads"],
                     gpu_kernel_ms=round(tp.eval_ms, 3), gpu_upload_ms=round(tp.prep_ms, 2),
                     gpu_download_ms=round(tp.reduce_ms, 2), gpu_call_s=round(gpu_call_s, 3),
                     kernel_GBps=round(nrec * 92 / (tp.eval_ms * 1e-3) / 1e9, 1),
@@ -495,7 +606,7 @@ def main():
     mixed = ctx.fetch(h, 0, case.nTime, case.nObs)
     partials = tm.n_partials
     items = tm.n_items
-    tune = ctx.get_tuning(acum.MIXED)
+    tune = ctx.case_tuning(h, 0, acum.MIXED)
 
     # post-processing of the mixed history on the GPU: 1/3-octave band levels from the resident pA
     note("tonal band spectra")
@@ -512,46 +623,53 @@ def main():
         fp64 = dict(value=round(evals / s64 / 1e9, 3), ms_per_step=round(s64 * 1e3, 3),
                     eval_ms=round(e64, 3), reduce_ms=round(r64, 4), partials=t64.n_partials,
                     roofline_frac=round(FLOP_PER_EVAL * evals / (e64 * 1e-3) / PEAK_FP64, 4))
+    ctx.release(h)
 
-    # CPU baseline + accuracy: FP64 oracle on the first observers (observers are independent)
-    nob = max(1, min(args.oracle_obs, case.nObs))
-    sub = acum.TonalCase()
-    C.pointer(sub)[0] = case
-    sub.nObs = nob
-    sub.precision = acum.FP64
-    threads = min(len(os.sched_getaffinity(0)), 16, nob)  # the oracle is parallel over observers
-    note(f"CPU oracle on {nob} observers")
-    t = time.perf_counter()
-    oT, oL, oA = O.tonal_f64(sub, threads)
-    cpu_s = time.perf_counter() - t
+    # CPU baseline + accuracy: the FP64 oracle on observers spread over the hemisphere (observers
+    # are independent, so the rate is exact for the sample), median of three runs; the float
+    # instance of the same restatement (the reference's float arithmetic) likewise
+    idx = P.spread_observers(case.nObs, args.oracle_obs)
+    sub = P.obs_subset(case, idx, acum.FP64)
+    nob = len(idx)
+    note(f"CPU oracle on {nob} observers, three runs")
+    (oT, oL, oA), cpu_s, cpu_runs = timed_runs(lambda: O.tonal_f64(sub, host["threads"]), 3)
     cpu_evals = sum(E) * case.nTime * nob
-    # the float instance of the same restatement (the reference's float arithmetic), same sample
-    note(f"CPU float oracle on {nob} observers")
-    sub.precision = acum.MIXED
-    t = time.perf_counter()
-    fT, fL, fA = O.tonal_f32(sub, threads)
-    cpu32_s = time.perf_counter() - t
+    note(f"CPU float oracle on {nob} observers, three runs")
+    sub32 = P.obs_subset(case, idx, acum.MIXED)
+    (fT, fL, fA), cpu32_s, _ = timed_runs(lambda: O.tonal_f32(sub32, host["threads"]), 3)
 
-    gA = mixed[2][:, :nob]
-    d_oaspl = np.abs(acum.oaspl(gA) - acum.oaspl(oA.astype(np.float32)))
-    acc = dict(observers=nob, mixed_vs_oracle_rel_l2_pA=O.rel_l2(gA, oA),
-               mixed_vs_oracle_max_doaspl_db=float(d_oaspl.max()),
+    dt = case.dPsi / case.oM
+    gm = [m[:, idx] for m in mixed]
+    a = tonal_accuracy(acum, O, P, gm, (oT, oL, oA), dt)
+    ref_spl, _ = P.band_levels(oA, dt)
+    acc = dict(observers=nob, mixed_vs_oracle_rel_l2_pA=a["rel_l2_pA"], mixed_vs_oracle_rel_l2_pT=a["rel_l2_pT"],
+               mixed_vs_oracle_rel_l2_pL=a["rel_l2_pL"], mixed_vs_oracle_max_doaspl_db=a["max_doaspl_db"],
+               mixed_vs_oracle_max_dband_db=a["max_dband_db"], mixed_worst_band_hz=a["worst_band_hz"],
+               mixed_gpu_spectra_vs_oracle_max_dband_db=P.band_error(sp_spl[idx], ref_spl)[0],
+               live_bands=a["live_bands"],
                float_oracle_vs_oracle_rel_l2_pA=O.rel_l2(fA, oA),
                float_oracle_vs_oracle_max_doaspl_db=float(
                    np.abs(acum.oaspl(fA) - acum.oaspl(oA.astype(np.float32))).max()))
     if fp64:
-        acc["fp64_vs_oracle_rel_l2_pA"] = O.rel_l2(plain[2][:, :nob], oA)
+        acc["fp64_vs_oracle_rel_l2_pA"] = O.rel_l2(plain[2][:, idx], oA)
+        acc["fp64_vs_oracle_rel_l2_pT"] = O.rel_l2(plain[0][:, idx], oT)
         d = np.abs(acum.oaspl(mixed[2]) - acum.oaspl(plain[2]))
         acc["mixed_vs_fp64_all_obs_max_doaspl_db"] = float(d.max())
 
-    o_spl, _ = O.tonal_spectra(mixed[2][:, :nob], case.dPsi / case.oM, iso)
+    o_spl, _ = O.tonal_spectra(mixed[2][:, idx], dt, iso)
     spectra = dict(kernels="spectra_stats + spectra_bins + spectra_bands", ms=round(sp_ms, 3), bands=int(iso.size),
-                   max_dspl_db_vs_oracle=float(np.abs(sp_spl[:nob] - o_spl).max()))
+                   max_dspl_db_vs_oracle=float(np.abs(sp_spl[idx] - o_spl).max()))
 
-    bpm = None
+    bpm = batched = stress = None
     if not args.no_bpm:
         note("broadband path")
-        bpm = bench_bpm(acum, O, ctx, args.steps)
+        bpm = leg_bpm(acum, O, ctx, args.steps, host)
+    if not args.no_batched:
+        note(f"batched leg: first {args.cases} cases, {args.wave} per launch")
+        batched = leg_batched(acum, O, P, ctx, host, args.cases, args.wave, 4, 2, not args.no_fp64)
+    if not args.no_stress:
+        note("stress leg")
+        stress = leg_stress(acum, O, P, ctx, host, 2, 16)
 
     flops = FLOP_PER_EVAL * evals
     achieved = flops / (eval_ms * 1e-3)
@@ -578,20 +696,22 @@ def main():
                    "tile": {"window": tune.window, "max_spread": tune.max_spread,
                             "max_tile": tune.max_tile, "motion_permille": tune.motion_permille,
                             "on_wider_motion_bound": bool(tm.motion_widened_cases)}},
-        # compute-bound ("mfma" in the roofline vocabulary: the arithmetic peak, not HBM); the
-        # arithmetic runs on the VALU (FP64 solve, FP32 integrand) and the matrix cores (the
-        # integrand's record products)
-        "roofline": {"bound": "mfma", "engines": "VALU FP64/FP32 + MFMA FP32 (record products)",
+        # compute-bound on the vector ALUs: the FP64 observer-time solve and the FP32 integrand
+        # (the integrand's four record products go through the matrix cores, 3 of ~73
+        # instructions per step); the peak is the FP32/FP64 blend of the 70/30 FLOP split
+        "roofline": {"bound": "valu", "engines": "VALU FP64/FP32 (+ MFMA FP32 for the record products)",
                      "achieved": round(achieved / 1e12, 3),
                      "peak": round(peak / 1e12, 3), "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": read_traffic(),
                      "kernel": "tonal_eval_kernel<mixed, moving>", "eval_ms": round(eval_ms, 3),
                      "flop_per_eval": FLOP_PER_EVAL},
         "cpu_baseline": {"value": round(cpu_evals / cpu_s / 1e9, 4), "unit": "G src-obs-time evals/s",
-                         "cores": threads, "kind": "port",
-                         "sample": f"FP64 oracle, same workload, first {nob} observers "
-                                   f"({cpu_evals:.3g} evals, {cpu_s:.1f} s)",
+                         "cores": host["threads"], "kind": "port", "runs": 3,
+                         "sample": f"FP64 oracle, same workload, {nob} observers spread over the hemisphere "
+                                   f"({cpu_evals:.3g} evals, median of 3 runs {cpu_s:.1f} s)",
+                         "run_seconds": [round(x, 2) for x in cpu_runs],
                          "float_instance_value": round(cpu_evals / cpu32_s / 1e9, 4)},
+        "host": host,
         "binning": {"kernel": "tonal_reduce_kernel<float>", "ms": round(red_ms, 4),
                     "bytes": bin_bytes, "GBps": round(bin_bytes / (red_ms * 1e-3) / 1e9, 1),
                     "frac_hbm": round(bin_bytes / (red_ms * 1e-3) / HBM_PEAK, 4),
@@ -601,15 +721,20 @@ def main():
         "accuracy": acc,
         "spectra": spectra,
         "bpm": bpm,
+        "batched": batched,
+        "stress": stress,
         "preprocess": prep,
         "work_items": items,
         "synth_s": round(synth_s, 2),
         "host_prep_s": round(host_prep_s, 2),
         "stage_s": round(stage_s, 2),
     }
-    ctx.release(h)
     ctx.close()
     print(json.dumps(out))
+    bad = parity_failures(out)
+    if bad:
+        print("parity bars broken: " + ", ".join(bad), file=sys.stderr)
+        return 3
     return 0
 
 
