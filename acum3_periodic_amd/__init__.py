@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libacum_hip.so")
+# ACUM_LIB_PATH selects another build of the same library (the sanitizer build of the host code)
+LIB_PATH = os.environ.get("ACUM_LIB_PATH") or os.path.join(_HERE, "lib", "libacum_hip.so")
 DRIVER_PATH = os.path.join(_HERE, "bin", "acum")
 
 ACUM_OK, ACUM_ERR_ARG, ACUM_ERR_IO, ACUM_ERR_DEVICE, ACUM_ERR_NOMEM, ACUM_ERR_NUMERIC, ACUM_ERR_PLAN = (

@@ -8,7 +8,8 @@ import numpy as np
 import acum3_periodic_amd as acum
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+# ACUM_ORACLE_PATH selects another build of the oracle (its sanitizer build)
+ORACLE_PATH = os.environ.get("ACUM_ORACLE_PATH") or os.path.join(ROOT, "oracle", "liboracle.so")
 REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libref_host.so")
 
 _f32p = C.POINTER(C.c_float)
