@@ -18,14 +18,14 @@
 
 namespace acum {
 hipError_t launch_preprocess(hipStream_t s, const PrepArgs& a, int grid);
-hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
+hipError_t launch_tonal_eval(bool mixed, bool moving, bool tail, int W, int grid, hipStream_t s,
                              const TonalCaseDev* cases, const Tile* tiles, const Item* items,
                              const int* qoff, int* counter);
 hipError_t eval_occupancy(bool mixed, bool moving, int W, int* blocks_per_cu);
 bool eval_window_supported(bool mixed, int W);
 hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* cases, int icase,
                                int nT, int nObs);
-hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, int nT, double SB, void* pos);
+hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, const TimeConst& k, void* pos);
 hipError_t launch_tonal_attr(bool mixed, hipStream_t s, const float* X, const float* N,
                              const float* V, const float* PR, int64_t n, double a0, double rhoRef,
                              void* attr, uint32_t* flags);
@@ -71,6 +71,7 @@ constexpr int kMotionPermille = 600;
 // [qoff[q], qoff[q+1]) of the launch's item range.
 struct Launch {
   bool moving = false;
+  bool tail = false;  // the variant with the tail for tent sides longer than two bins
   int window = 0;
   int item0 = 0, nitems = 0;
   int grid = 0;
@@ -314,8 +315,112 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   k.ux = vx / k.KB;
   k.uy = vy / k.KB;
   k.uz = vz / k.KB;
+  k.h2 = 1.0 / (2.0 * k.C2);
   k.still = (vx == 0.0 && vy == 0.0 && vz == 0.0) ? 1 : 0;
   return k;
+}
+
+// A source on an observer: an observer within kCoincident metres of a source position at any
+// step (|obs - src|^2 formed in FP64 from the parsed floats, as the oracle does) fails the case
+// with ACUM_ERR_NUMERIC.  The canonical observer time expands |obs - src|^2 about the sources'
+// centre, whose rounding does not give an exact zero there.  Bounding boxes (all sources, each
+// surface, each element) leave only observers near a source to the position-by-position test.
+constexpr double kCoincident = 1e-6;
+
+int check_coincident(acum_ctx* ctx, const StagedCase& sc, const float* rObs) {
+  const std::vector<float>& P = *sc.pos;
+  const int64_t nT = sc.nT;
+  const double r2 = kCoincident * kCoincident;
+  struct Box {
+    double lo[3], hi[3];
+  };
+  auto near = [&](const Box& b, const double* o) {
+    double d2 = 0;
+    for (int d = 0; d < 3; d++) {
+      const double g = std::max({b.lo[d] - o[d], 0.0, o[d] - b.hi[d]});
+      d2 += g * g;
+    }
+    return d2 <= 4.0 * r2;
+  };
+  std::vector<Box> eb(sc.nE);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < sc.nE; e++) {
+    Box b{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+    for (int64_t k = 0; k < nT; k++)
+      for (int d = 0; d < 3; d++) {
+        const double v = P[(e * nT + k) * 3 + d];
+        b.lo[d] = std::min(b.lo[d], v);
+        b.hi[d] = std::max(b.hi[d], v);
+      }
+    eb[e] = b;
+  }
+  const size_t ns = sc.surf0.size();
+  std::vector<Box> sb(ns);
+  Box all{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+  for (size_t s = 0; s < ns; s++) {
+    const int64_t a = sc.surf0[s], b = (s + 1 < ns) ? sc.surf0[s + 1] : sc.nE;
+    Box x{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+    for (int64_t e = a; e < b; e++)
+      for (int d = 0; d < 3; d++) {
+        x.lo[d] = std::min(x.lo[d], eb[e].lo[d]);
+        x.hi[d] = std::max(x.hi[d], eb[e].hi[d]);
+      }
+    sb[s] = x;
+    for (int d = 0; d < 3; d++) {
+      all.lo[d] = std::min(all.lo[d], x.lo[d]);
+      all.hi[d] = std::max(all.hi[d], x.hi[d]);
+    }
+  }
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(| : bad)
+  for (int64_t i = 0; i < sc.nObs; i++) {
+    const double o[3] = {rObs[3 * i], rObs[3 * i + 1], rObs[3 * i + 2]};
+    if (!near(all, o)) continue;
+    for (size_t s = 0; s < ns && !bad; s++) {
+      if (!near(sb[s], o)) continue;
+      const int64_t a = sc.surf0[s], b = (s + 1 < ns) ? sc.surf0[s + 1] : sc.nE;
+      for (int64_t e = a; e < b && !bad; e++) {
+        if (!near(eb[e], o)) continue;
+        for (int64_t k = 0; k < nT; k++) {
+          const float* q = &P[(e * nT + k) * 3];
+          const double dx = o[0] - (double)q[0], dy = o[1] - (double)q[1], dz = o[2] - (double)q[2];
+          if (std::fma(dz, dz, std::fma(dy, dy, dx * dx)) < r2) {
+            bad = 1;
+            break;
+          }
+        }
+      }
+    }
+  }
+  if (bad) return ctx_fail(ctx, ACUM_ERR_NUMERIC, "an observer lies on a source (within 1e-6 m of a source position)");
+  return ACUM_OK;
+}
+
+// Origin of the observer-time expansion: the centre of the bounding box of every source position
+// (host copy [nE][nT][3] of the parsed floats); the oracle takes the same from the case arrays.
+void set_origin(StagedCase& sc) {
+  const std::vector<float>& P = *sc.pos;
+  const int64_t n = (int64_t)P.size() / 3;
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma omp parallel
+  {
+    double l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma omp for schedule(static) nowait
+    for (int64_t i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) {
+        const double v = P[3 * i + d];
+        l[d] = std::min(l[d], v);
+        h[d] = std::max(h[d], v);
+      }
+#pragma omp critical
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], l[d]);
+      hi[d] = std::max(hi[d], h[d]);
+    }
+  }
+  sc.k.Ox = (lo[0] + hi[0]) * 0.5;
+  sc.k.Oy = (lo[1] + hi[1]) * 0.5;
+  sc.k.Oz = (lo[2] + hi[2]) * 0.5;
 }
 
 int validate(acum_ctx* ctx, const acum_tonal_case* c) {
@@ -381,6 +486,9 @@ int stage_case(acum_ctx* ctx, const acum_tonal_case* c, StagedCase& sc) {
       e0 += c->E[s];
     }
   }
+  set_origin(sc);
+  rc = check_coincident(ctx, sc, c->rObs);
+  if (rc) return rc;
   HIPCHK(ctx, hipMalloc(&sc.d_obs, (size_t)c->nObs * 3 * sizeof(float)));
   HIPCHK(ctx, hipMemcpyAsync(sc.d_obs, c->rObs, (size_t)c->nObs * 3 * sizeof(float),
                              hipMemcpyHostToDevice, ctx->stream));
@@ -464,6 +572,9 @@ int stage_raw_run(acum_ctx* ctx, const acum_run* r, StagedCase& sc) {
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < q; i++)
     for (int d = 0; d < 3; d++) (*sc.pos)[i * 3 + d] = (float)((double)c.X[i * 4 + d] * C);
+  set_origin(sc);
+  rc = check_coincident(ctx, sc, c.rObs);
+  if (rc) return rc;
   HIPCHK(ctx, hipMalloc(&sc.d_obs, (size_t)c.nObs * 3 * sizeof(float)));
   HIPCHK(ctx, hipMemcpyAsync(sc.d_obs, c.rObs, (size_t)c.nObs * 3 * sizeof(float), hipMemcpyHostToDevice,
                              ctx->stream));
@@ -480,7 +591,7 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
   HIPCHK(ctx, launch_reset(ctx->stream, ctx->d_counter, ctx->d_flags));
   if (!sc.d_pos) {
     HIPCHK(ctx, hipMalloc(&sc.d_pos, (size_t)n * sizeof(PosRec)));
-    HIPCHK(ctx, launch_tonal_pos(ctx->stream, sc.d_aos[0], n, (int)sc.nT, sc.k.SB, sc.d_pos));
+    HIPCHK(ctx, launch_tonal_pos(ctx->stream, sc.d_aos[0], n, sc.k, sc.d_pos));
   }
   const size_t bytes = (size_t)n * (mixed ? sizeof(AttrT<float>) : sizeof(AttrT<double>));
   HIPCHK(ctx, hipMalloc(&sc.d_attr[prec], bytes));
@@ -645,27 +756,35 @@ int build_plan(acum_ctx* ctx, Stage& st, int prec) {
   const bool mixed = prec == ACUM_MIXED;
   // one launch per kernel variant (still air or moving medium, window) in use; a case's
   // capacity is that of its own variant, so it gets the same groups alone or in a batch
-  std::vector<std::pair<int, int>> kinds;  // (moving, window)
+  // A case whose observer time can advance by more than two bins in one step needs the kernel
+  // variant with the tail for long tent sides; the bound is rigorous: |d tau / d x| <=
+  // r2b / (1 - Minf) bins per metre of source motion (the convected delay's gradient), so a step
+  // advances by at most SB + (largest source displacement per step) r2b / (1 - Minf).
+  std::vector<std::tuple<int, int, int>> kinds;  // (moving, window, tail)
   std::vector<int> kind_of(nc), capacity(nc);
   for (int c = 0; c < nc; c++) {
-    const StagedCase& sc = st.cases[c];
+    StagedCase& sc = st.cases[c];
     const int W = tunes[c].window;
     if (!eval_window_supported(mixed, W)) return ctx_fail(ctx, ACUM_ERR_ARG, "no eval kernel for this window size");
     int bpc = 0;
     HIPCHK(ctx, eval_occupancy(mixed, !sc.k.still, W, &bpc));
     if (bpc < 1) return ctx_fail(ctx, ACUM_ERR_DEVICE, "eval kernel does not fit a compute unit");
     capacity[c] = bpc * ctx->cu_count;
-    kinds.emplace_back(sc.k.still ? 0 : 1, W);
+    element_steps(sc);
+    const double smax = sc.step->empty() ? 0.0 : *std::max_element(sc.step->begin(), sc.step->end());
+    const bool tail = !(sc.k.SB + smax * sc.k.r2b / (1.0 - sc.Minf) <= 1.99);
+    kinds.emplace_back(sc.k.still ? 0 : 1, W, tail ? 1 : 0);
   }
   {
-    std::vector<std::pair<int, int>> u = kinds;
+    std::vector<std::tuple<int, int, int>> u = kinds;
     std::sort(u.begin(), u.end());
     u.erase(std::unique(u.begin(), u.end()), u.end());
     for (int c = 0; c < nc; c++) kind_of[c] = (int)(std::lower_bound(u.begin(), u.end(), kinds[c]) - u.begin());
     p.launches.assign(u.size(), Launch());
     for (size_t l = 0; l < u.size(); l++) {
-      p.launches[l].moving = u[l].first != 0;
-      p.launches[l].window = u[l].second;
+      p.launches[l].moving = std::get<0>(u[l]) != 0;
+      p.launches[l].window = std::get<1>(u[l]);
+      p.launches[l].tail = std::get<2>(u[l]) != 0;
     }
   }
   std::vector<std::vector<Tile>> ct(nc);
@@ -852,7 +971,7 @@ int enqueue_pass(acum_ctx* ctx, Stage& st, int prec, bool with_events) {
   for (int l = 0; l < nl; l++) {
     const Launch& L = p.launches[l];
     if (L.nitems > 0)
-      HIPCHK(ctx, launch_tonal_eval(mixed, L.moving, L.window, L.grid, s, p.d_cases, p.d_tiles,
+      HIPCHK(ctx, launch_tonal_eval(mixed, L.moving, L.tail, L.window, L.grid, s, p.d_cases, p.d_tiles,
                                     p.d_items + L.item0, p.d_qoff + (size_t)l * (kQueues + 1),
                                     p.d_counter + (size_t)l * kQueues));
   }
@@ -1034,7 +1153,7 @@ int tune_case(acum_ctx* ctx, const StagedCase& sc, int prec, TuneEntry* out, int
   for (int W : {16, 20, 24}) {
     if (!eval_window_supported(mixed, W)) continue;
     for (int sp : {2, 3, 4})
-      for (int pm : {600, 800}) {
+      for (int pm : {500, 600, 800}) {
         acum_tune t = keep;
         t.window = W;
         t.max_spread = sp;

@@ -6,41 +6,53 @@ namespace acum {
 
 // Element-time records: one (element, source step), wave-uniform in the tonal kernel, read with
 // scalar loads, element-major ([element][step]) like the reference AoS (io.cu:280-284).
-// PosRec holds the source position in FP64 (the exact parsed float) for the canonical observer
-// time and jt = j (dTau oM / dPsi), the step's emission time in bins.  AttrT holds the
-// observer-independent factors of the Farassat 1A integrand (core.cu:68-154), formed in FP64
-// from the parsed floats and stored in the path's precision R:
+// PosRec holds what the canonical observer time (below) needs of the source position, formed in
+// FP64 from the parsed floats, and the parsed floats themselves (the FP32 radiation vector).
+// AttrT holds the observer-independent factors of the Farassat 1A integrand (core.cu:68-154),
+// formed in FP64 from the parsed floats and stored in the path's precision R:
 //   v   = V / a0,  vd = dV/dt / a0                        (Mr, dMr of core.cu:93-95)
-//   l   = P n,     ld = dP/dt n + P dn/dt                  (L, dL/dt of core.cu:130-131)
-//   aM1 = a0 (M^2 - 1) = |V|^2 / a0 - a0,  aLM = a0 L_M = P (V.n)   (core.cu:94,134)
+//   l   = cL P n,  ld = cL (dP/dt n + P dn/dt)             (L, dL/dt of core.cu:130-131)
+//   aM1 = a0 (M^2 - 1) = |V|^2 / a0 - a0,  aLM = cL a0 L_M = cL P (V.n)   (core.cu:94,134)
 //   cT  = rhoRef (V.n) dS a0 D1,  cL = dS D1,  D1 = 1/(4 pi a0)   (core.cu:53,123-124,138-141)
+// (the loading factor cL is folded into l, ld and aLM, which only enter the loading term)
 struct PosRec {
-  double x, y, z, jt;
-};  // 32 bytes
+  double mx, my, mz;  // (-2 C2) (p - O)
+  double q;           // C2 |p - O|^2
+  double wp;          // w.(p - O) (zero in still air)
+  double jt;          // j SB: the step's emission time in bins
+  float x, y, z;      // the parsed position
+  float pad;
+};  // 64 bytes
+// The part of a PosRec the observer time reads (the first 48 bytes).
+struct PosSolve {
+  double mx, my, mz, q, wp, jt;
+};
 template <typename R>
 struct AttrT {
   R v[3], vd[3], l[3], ld[3];
-  R aM1, aLM, cT, cL;
+  R aM1, aLM, cT, unused;
 };  // 64 bytes (float), 128 bytes (double)
 
 // Canonical FP64 observer time, in units of the observer-time bin dPsi; the CPU oracle forms
-// the same values with the same operations (no contraction other than the explicit fma).
-//   d = obs - src;  c = fma(dz,dz, fma(dy,dy, dx*dx))                 |obs - src|^2
-//   still air:  s = sqrt(c);  tau = fma(s, r2b, jt)                  r = s
-//   moving:     b = fma(dz,wz, fma(dy,wy, dx*wx));  T = sqrt(fma(b, b, c*C2)) - b;
-//               tau = T + jt                                          r = aB T
-// with r2b = (oM / a0) / dPsi, KB = oM / dPsi, jt = j (dTau KB), na = a0^2 - |vinf|^2 > 0,
-// w = vinf (KB / na), C2 = KB (KB / na), aB = a0 / KB.  T = KB t is the delay in bins, t the
-// positive root (-b' - sqrt(b'^2 - 4 a' c)) / (2 a') of a' t^2 + b' t + c = 0, a' = |vinf|^2 -
-// a0^2 < 0, b' = -2 d.vinf (acous_dist, core.cu:261-279): t = (s - d.vinf) / na with
-// s = sqrt((d.vinf)^2 + c na), so KB t = sqrt((KB d.vinf / na)^2 + c KB^2 / na) - KB d.vinf / na.
+// the same values with the same operations (no contraction other than the explicit fma).  With
+// KB = oM / dPsi, na = a0^2 - |vinf|^2 > 0, C2 = KB (KB / na), w = vinf (KB / na), SB = dTau KB,
+// aB = a0 / KB, u = vinf / KB and the origin O = the centre of the sources' bounding box:
+//   per observer: o = obs - O, qo = C2 fma(oz,oz, fma(oy,oy, ox ox)), wo = fma(oz,wz, fma(oy,wy, ox wx))
+//   per eval:     cc = fma(ox,mx, fma(oy,my, fma(oz,mz, qo + q)))        = C2 |obs - src|^2
+//                 b = wo - wp;  T = sqrt(fma(b, b, cc)) - b;  tau = T + jt
+//                 (still air: T = sqrt(cc));  r = aB T;  r_vec = (obs - src) - u T
+// T = KB t is the delay in bins, t the positive root of a' t^2 + b' t + c = 0 with
+// a' = |vinf|^2 - a0^2 < 0, b' = -2 d.vinf, c = |d|^2 (acous_dist, core.cu:261-279):
+// t = (s - d.vinf) / na with s = sqrt((d.vinf)^2 + c na), scaled by KB inside the square root.
 // tau is (j dTau + r / a0) oM / dPsi (core.cu:156).  The bins of an eval are the integers b in
 // (tau_prev, tau] (rising side of the tent) and in (tau, tau_next) (falling side); bin b is
 // observer time b dPsi, stored at b mod nT.
 struct TimeConst {
   double a0, r2b, KB, SB;
   double wx, wy, wz, C2;  // moving medium: canonical observer time
-  double aB, ux, uy, uz;  // moving medium: r = aB T, r_vec = d - u T with u = vinf / KB
+  double aB, ux, uy, uz;  // r = aB T, r_vec = d - u T with u = vinf / KB
+  double Ox, Oy, Oz;      // origin of the expansion: centre of the sources' bounding box
+  double h2;              // 1 / (2 C2): p - O = -h2 m, for the FP64 radiation vector
   int32_t nT, still;      // still: vinf == 0
 };
 

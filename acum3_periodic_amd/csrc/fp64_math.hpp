@@ -26,6 +26,22 @@ __device__ __forceinline__ double sqrt_pos(double x) {
   return g;
 }
 
+// The same with one residual correction: before its final rounding the result is within about
+// 2^-92 (relative) of sqrt(x), so it is the correctly rounded root except where sqrt(x) lies
+// that close to a rounding boundary.  Used where the square root feeds FP32 arithmetic (the
+// mixed path's observer time, whose weights are FP32): there the second correction changes
+// nothing measurable.
+__device__ __forceinline__ double sqrt_pos1(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  double h = 0.5 * y;
+  const double r = __builtin_fma(-g, h, 0.5);
+  g = __builtin_fma(g, r, g);
+  h = __builtin_fma(h, r, h);
+  const double d = __builtin_fma(-g, g, x);
+  return __builtin_fma(d, h, g);
+}
+
 // Reciprocal of a nonzero normal double to about 1 ulp: hardware estimate plus two
 // Newton-Raphson steps e = 1 - x y, y += y e.
 __device__ __forceinline__ double rcp_nr(double x) {

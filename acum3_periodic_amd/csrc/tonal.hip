@@ -25,6 +25,7 @@
 // does everything in FP64.  The FP64 observer-time/bin code is the canonical form the CPU
 // oracle uses, compiled without contraction, so bin indices agree bit for bit.
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <cstdint>
 #include <type_traits>
 #include "dev_types.hpp"
@@ -33,8 +34,9 @@
 namespace acum {
 
 // Records are read with wave-uniform addresses.  Through constant-address-space pointers the
-// reads are scalar loads into SGPRs (the mixed path); through global-address-space pointers the
-// compiler may also use vector loads, which keeps the larger FP64 records out of the SGPR file.
+// reads are scalar loads into SGPRs (positions, and the mixed path's attributes); through
+// global-address-space pointers the compiler may also use vector loads, which keeps the larger
+// FP64 attribute records out of the SGPR file.
 template <typename T>
 using cptr = const __attribute__((address_space(1))) T*;
 template <typename T>
@@ -67,23 +69,39 @@ __device__ __forceinline__ T load_words(kptr<T> p) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Canonical FP64 observer time in bins (acous_dist core.cu:261-279, ttemp core.cu:156); the CPU
-// oracle evaluates the same operations in the same order.  `delay` returns r (still air, metres)
-// or T = KB r / a0 (moving medium, the delay in bins).
-template <bool MOVING>
-__device__ __forceinline__ double obs_time(const TimeConst& k, double jt, double dx, double dy,
-                                           double dz, double* delay) {
-  const double c = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+// Canonical FP64 observer time in bins (acous_dist core.cu:261-279, ttemp core.cu:156; the form
+// is described with TimeConst); the CPU oracle evaluates the same operations in the same order.
+// `delay` returns T, the delay in bins.  EXACT selects the correctly rounded square root (the FP64
+// path and the diagnostic bin lookup); the mixed path's weights are FP32, and there one residual
+// correction of the root suffices.
+struct LaneObs {
+  double ox, oy, oz;  // observer - O
+  double qo, wo;      // C2 |o|^2, w.o
+};
+
+__device__ __forceinline__ LaneObs lane_obs(const TimeConst& k, double obx, double oby, double obz) {
+  LaneObs L;
+  L.ox = obx - k.Ox;
+  L.oy = oby - k.Oy;
+  L.oz = obz - k.Oz;
+  L.qo = __builtin_fma(L.oz, L.oz, __builtin_fma(L.oy, L.oy, L.ox * L.ox)) * k.C2;
+  L.wo = __builtin_fma(L.oz, k.wz, __builtin_fma(L.oy, k.wy, L.ox * k.wx));
+  return L;
+}
+
+template <bool MOVING, bool EXACT, typename P>
+__device__ __forceinline__ double obs_time(const LaneObs& L, const P& p, double* delay) {
+  const double cc = __builtin_fma(L.ox, p.mx, __builtin_fma(L.oy, p.my, __builtin_fma(L.oz, p.mz, L.qo + p.q)));
+  double T;
   if (!MOVING) {
-    const double s = sqrt_pos(c);
-    *delay = s;
-    return __builtin_fma(s, k.r2b, jt);
+    T = EXACT ? sqrt_pos(cc) : sqrt_pos1(cc);
   } else {
-    const double b = __builtin_fma(dz, k.wz, __builtin_fma(dy, k.wy, dx * k.wx));
-    const double T = sqrt_pos(__builtin_fma(b, b, c * k.C2)) - b;
-    *delay = T;
-    return T + jt;
+    const double b = L.wo - p.wp;
+    const double x = __builtin_fma(b, b, cc);
+    T = (EXACT ? sqrt_pos(x) : sqrt_pos1(x)) - b;
   }
+  *delay = T;
+  return T + p.jt;
 }
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0; }
@@ -118,27 +136,26 @@ struct Prec<true> {
 //   pThick = rhoRef Vn dS a0 (dMr T2 + a0 (Mr - M^2) T4)
 //   pLoad  = dS (dLr T1 + Lr dMr T2 + a0 (Lr - L_M) T3 + a0 Lr (Mr - M^2) T4)
 // with T1 = D1/(r (1-Mr)^2), T2 = T1/(1-Mr), T3 = T1/r, T4 = T3/(1-Mr).  With the record
-// products (AttrT) dotted into r_vec -- Mr = dV/r, dMr = dVd/r, Lr = dL/r, dLr = dLd/r for
-// dV = v.r_vec, dVd = vd.r_vec, dL = l.r_vec, dLd = ld.r_vec -- and g = 1/(r (1 - Mr)) =
-// 1/(r - dV), every 1/r cancels:
-//   pThick = cT g^3 (r (dVd - a0 M^2) + a0 dV) = cT g^2 (g r w - a0)
-//   pLoad  = cL g^2 (dL g (dVd - a0 M^2 + a0) + dLd - a0 L_M) = cL g^2 (dL g w + dLd - a0 L_M)
-// with w = dVd - a0 (M^2 - 1) (using 1 + g dV = r g and g (r - dV) = 1), one reciprocal per
-// eval.  d is obs - src; `del` is r in still air and the delay T in bins in a moving medium,
-// where r = aB T and r_vec = d - u T (core.cu:86; aB = a0 / KB, u = vinf / KB).  Returns r (1 - Mr):
-// positive for a valid eval, zero for a source on the observer (r = 0 gives r_vec = 0 and dV = 0)
-// and negative for Mr > 1.
+// products (AttrT) dotted into r_vec -- Mr = dV/r, dMr = dVd/r, Lr = dL/(cL r), dLr = dLd/(cL r)
+// for dV = v.r_vec, dVd = vd.r_vec, dL = l.r_vec, dLd = ld.r_vec (l, ld carry the factor cL) --
+// and g = 1/(r (1 - Mr)) = 1/(r - dV), every 1/r cancels:
+//   pThick = cT g^3 (r (dVd - a0 M^2) + a0 dV) = cT g^2 (r t - a0)
+//   pLoad  = g^2 (dL g (dVd - a0 M^2 + a0) + dLd - cL a0 L_M) = g^2 (dL t + dLd - aLM)
+// with w = dVd - a0 (M^2 - 1) and t = g w (using 1 + g dV = r g and g (r - dV) = 1), one
+// reciprocal per eval.  d is obs - src and `del` the delay T in bins: r = aB T and r_vec = d - u T (core.cu:86;
+// aB = a0 / KB, u = vinf / KB; u = 0 in still air).  Returns r (1 - Mr): positive for a valid
+// eval, zero for a source on the observer (r = 0 gives r_vec = 0 and dV = 0) and negative for
+// Mr > 1.
 // r and r_vec of an eval
 template <typename R, bool MOVING>
 __device__ __forceinline__ void radiation_vector(R dx, R dy, R dz, R del, R aB, R vx, R vy, R vz,
                                                  R* r, R* rx, R* ry, R* rz) {
+  *r = aB * del;
   if (MOVING) {
-    *r = aB * del;
     *rx = fmaT(-vx, del, dx);
     *ry = fmaT(-vy, del, dy);
     *rz = fmaT(-vz, del, dz);
   } else {
-    *r = del;
     *rx = dx;
     *ry = dy;
     *rz = dz;
@@ -147,14 +164,14 @@ __device__ __forceinline__ void radiation_vector(R dx, R dy, R dz, R del, R aB, 
 
 // the integrand from r and the four record products
 template <typename R>
-__device__ __forceinline__ R integrand_tail(R aM1, R aLM, R cT, R cL, R r, R dV, R dVd, R dL,
-                                            R dLd, R a0, R* pth, R* plo) {
+__device__ __forceinline__ R integrand_tail(R aM1, R aLM, R cT, R r, R dV, R dVd, R dL, R dLd, R a0,
+                                            R* pth, R* plo) {
   const R rm = r - dV;                // r (1 - Mr)
   const R g = rcpT(rm);
   const R g2 = g * g;
-  const R w = dVd - aM1;
-  *pth = (cT * g2) * fmaT(g * r, w, -a0);
-  *plo = (cL * g2) * fmaT(dL * g, w, dLd - aLM);
+  const R t = g * (dVd - aM1);        // g w, shared by both terms
+  *pth = (cT * g2) * fmaT(r, t, -a0);
+  *plo = g2 * fmaT(dL, t, dLd - aLM);
   return rm;
 }
 
@@ -167,7 +184,7 @@ __device__ __forceinline__ R integrand(const AttrT<R>& q, R dx, R dy, R dz, R de
   const R dVd = fmaT(q.vd[2], rz, fmaT(q.vd[1], ry, q.vd[0] * rx));
   const R dL = fmaT(q.l[2], rz, fmaT(q.l[1], ry, q.l[0] * rx));
   const R dLd = fmaT(q.ld[2], rz, fmaT(q.ld[1], ry, q.ld[0] * rx));
-  return integrand_tail<R>(q.aM1, q.aLM, q.cT, q.cL, r, dV, dVd, dL, dLd, a0, pth, plo);
+  return integrand_tail<R>(q.aM1, q.aLM, q.cT, r, dV, dVd, dL, dLd, a0, pth, plo);
 }
 
 // Mixed path: the four record products of 64 observers by the matrix cores.  With
@@ -182,7 +199,7 @@ struct RecRow {
   float x, y, z;
 };
 struct RecTail {
-  float aM1, aLM, cT, cL;
+  float aM1, aLM, cT, unused;
 };
 struct MixRec {
   RecTail t;
@@ -206,7 +223,7 @@ __device__ __forceinline__ float integrand_mfma(const MixRec& q, float dx, float
   d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.x, rx, d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.y, ry, d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.z, rz, d, 0, 0, 0);
-  return integrand_tail<float>(q.t.aM1, q.t.aLM, q.t.cT, q.t.cL, r, d[0], d[1], d[2], d[3], a0, pth, plo);
+  return integrand_tail<float>(q.t.aM1, q.t.aLM, q.t.cT, r, d[0], d[1], d[2], d[3], a0, pth, plo);
 }
 
 // The high word of a finite value read as a signed integer is positive exactly when the value is
@@ -215,6 +232,13 @@ __device__ __forceinline__ float integrand_mfma(const MixRec& q, float dx, float
 // are caught from the outputs by the reduction kernel.
 __device__ __forceinline__ int sign_word(float x) { return __float_as_int(x); }
 __device__ __forceinline__ int sign_word(double x) { return (int)(__double_as_longlong(x) >> 32); }
+// one v_min3_i32 per step (the compiler's reassociation across the two-step loop made three
+// instructions of two steps' minimums)
+__device__ __forceinline__ int min3i(int a, int b, int c) {
+  int r;
+  __asm__("v_min3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
 
 // ---------------------------------------------------------------------------------------------
 // Eval + binning.  One wave per workgroup, persistent over a queue of items.
@@ -239,7 +263,7 @@ constexpr int window_waves(bool mixed, int W) {
   return per_cu / 4 > 4 ? 4 : (per_cu / 4 < 1 ? 1 : per_cu / 4);
 }
 
-template <bool MIXED, bool MOVING, int W>
+template <bool MIXED, bool MOVING, bool TAIL, int W>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(window_waves(MIXED, W))))
 void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
                                                         const Tile* __restrict__ tiles,
@@ -253,8 +277,13 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
   // what a step reads of its attribute record: the whole record (FP64), or the scalar fields
   // and the lane's row for the matrix-core products (mixed)
   using Rec = std::conditional_t<MIXED, MixRec, Attr>;
-  using PosPtr = typename P::template ptr<PosRec>;
+  // positions always through scalar loads (the FP64 path's went through vector loads, 25% slower
+  // there: a wave-uniform record has no use for 64 lanes of vector addressing); the FP64 path's
+  // 128-byte attribute records stay in vector registers, out of the SGPR file
+  using PosPtr = kptr<PosRec>;
   using AttrPtr = typename P::template ptr<Attr>;
+  // the mixed path's weights are FP32: one residual correction of the observer time's root
+  constexpr bool EXACT = !MIXED;
   // win[slot][lane] = (thickness, loading) partial of one observer-time bin
   __shared__ R2 win[W * 64];
   const int lane = threadIdx.x;
@@ -280,20 +309,31 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
     }
     const Item item = items[idx];
     const TonalCaseDev* Cp = cases + item.icase;
-    TimeConst k = Cp->k;
-    // keep the observer-time factors in vector registers: an FMA may read one scalar operand,
-    // and the other is the record's jt
-    __asm__("" : "+v"(k.r2b));
+    const TimeConst k = Cp->k;
     const int nT = k.nT, nObs = Cp->nObs;
     const int o = item.obs0 + lane;
     const bool active = o < nObs;
     const int oc = active ? o : nObs - 1;
-    const double ox = Cp->obs[3 * oc], oy = Cp->obs[3 * oc + 1], oz = Cp->obs[3 * oc + 2];
+    const float ofx = Cp->obs[3 * oc], ofy = Cp->obs[3 * oc + 1], ofz = Cp->obs[3 * oc + 2];
+    const LaneObs L = lane_obs(k, ofx, ofy, ofz);
     R* slab = (R*)Cp->slab + ((int64_t)item.group * nObs + oc) * (int64_t)nT * 2;
     const PosPtr pos = (PosPtr)(uintptr_t)Cp->pos;
     const AttrPtr attr = (AttrPtr)(uintptr_t)Cp->attr;
     const R a0 = (R)k.a0, aB = (R)k.aB, ux = (R)k.ux, uy = (R)k.uy, uz = (R)k.uz;
     const double dnT = (double)nT;
+    // obs - src of the radiation vector: in FP32 from the parsed floats (mixed; the same bits as
+    // the FP64 difference rounded to float), in FP64 as o + h2 m (FP64 path)
+    auto diff = [&](const PosRec& p, R* dx, R* dy, R* dz) __attribute__((always_inline)) {
+      if constexpr (MIXED) {
+        *dx = ofx - p.x;
+        *dy = ofy - p.y;
+        *dz = ofz - p.z;
+      } else {
+        *dx = __builtin_fma(p.mx, k.h2, L.ox);
+        *dy = __builtin_fma(p.my, k.h2, L.oy);
+        *dz = __builtin_fma(p.mz, k.h2, L.oz);
+      }
+    };
     for (int ti = item.tile0; ti < item.tile0 + item.ntile; ti++) {
       const Tile tl = tiles[ti];
       for (int k0 = 0; k0 < nT; k0 += tl.K) {
@@ -303,21 +343,26 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
         int base;
         // the two positions an element's pass starts from (steps k0-1 and k0); the next
         // element's are loaded (through vector registers) while this element's steps run
-        PosRec nm = load_words<PosRec>(pos + tl.e0 * nT + km0), nc = load_words<PosRec>(pos + tl.e0 * nT + k0);
+        const cptr<PosRec> vpos = (cptr<PosRec>)(uintptr_t)Cp->pos;
+        // the solve part of record i (the first 48 bytes of its 64)
+        auto solve_part = [&](int64_t i) __attribute__((always_inline)) {
+          return load_words<PosSolve>((cptr<PosSolve>)(vpos + i));
+        };
+        PosSolve nm = solve_part(tl.e0 * nT + km0);
+        PosRec nc = load_words<PosRec>(vpos + tl.e0 * nT + k0);
         {
-          const PosRec& p = nm;
           double d;
-          double tb = obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &d);
+          double tb = obs_time<MOVING, EXACT>(L, nm, &d);
           if (k0 == 0) tb = tb - dnT;
           base = (int)floor(tb) - tl.spread - 1;
         }
-        const cptr<PosRec> vpos = (cptr<PosRec>)(uintptr_t)Cp->pos;
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
           const PosPtr pe = pos + e * nT;
           const AttrPtr ae = attr + e * nT;
-          const PosRec pm_e = nm, pc_e = nc;
+          const PosSolve pm_e = nm;
+          const PosRec pc_e = nc;
           if (e + 1 < tl.e0 + tl.ne) {
-            nm = load_words<PosRec>(vpos + (e + 1) * nT + km0);
+            nm = solve_part((e + 1) * nT + km0);
             nc = load_words<PosRec>(vpos + (e + 1) * nT + k0);
           }
           // Per-step state, kept twice and used in turn (cur is the step being evaluated, prv the
@@ -332,15 +377,10 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
           R dxc, dyc, dzc, delc;  // obs - src and the delay of the step being evaluated
           {
             double dl;
-            const PosRec& p = pm_e;
-            double tp = obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &dl);
+            double tp = obs_time<MOVING, EXACT>(L, pm_e, &dl);
             if (k0 == 0) tp = tp - dnT;  // the step before step 0 is step nT-1 of the previous period
-            const PosRec& q = pc_e;
-            const double dx = ox - q.x, dy = oy - q.y, dz = oz - q.z;
-            const double tc = obs_time<MOVING>(k, q.jt, dx, dy, dz, &dl);
-            dxc = (R)dx;
-            dyc = (R)dy;
-            dzc = (R)dz;
+            const double tc = obs_time<MOVING, EXACT>(L, pc_e, &dl);
+            diff(pc_e, &dxc, &dyc, &dzc);
             delc = (R)dl;
             const R dp = (R)(tc - tp);
             s1 = Step{tp, (int)floor(tp), (R)0};
@@ -351,7 +391,8 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
           }
           winmask |= __ballot(s0.ic - 1 - base < 0);  // the pass's first footprint
           // lanes whose current step has a rising side longer than two bins
-          uint64_t longp = __ballot(s1.ic < s0.ic - 1 && s0.t - s1.t > 1.999);
+          uint64_t longp = 0;
+          if constexpr (TAIL) longp = __ballot(s1.ic < s0.ic - 1 && s0.t - s1.t > 1.999);
           // record pipeline: positions one step ahead of the solve, attributes one step ahead
           // of the integrand, each with one load in flight
           PosRec p0 = load_words<PosRec>(pe + kn0), p1;
@@ -381,9 +422,8 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             // overlaps the solve and the integrand instead of the window update
             __builtin_amdgcn_sched_barrier(0);
             // observer time of the next step
-            const double dxn = ox - pn.x, dyn = oy - pn.y, dzn = oz - pn.z;
             double dln;
-            double t_next = obs_time<MOVING>(k, pn.jt, dxn, dyn, dzn, &dln);
+            double t_next = obs_time<MOVING, EXACT>(L, pn, &dln);
             if (wrap) t_next = t_next + dnT;
             // integrand of this step
             R pth, plo;
@@ -395,7 +435,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             // tent weights of the footprint
             const R fr = fractT(cur.t, R());  // tau - ic, exact
             const R dn = (R)(t_next - cur.t);
-            valid = min(valid, min(sign_word(rm), sign_word(dn)));
+            valid = min3i(valid, sign_word(rm), sign_word(dn));
             const R idp = cur.id, idn = rcpT(dn);
             const R frm = fr - (R)1;
             const R w0 = clamp01(fmaT(-fr, idp, (R)1 - idp));  // bin ic-1: 1 - (fr + 1) / dp
@@ -414,42 +454,42 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             wp[64] = v1;
             wp[128] = v2;
             wp[192] = v3;
-            // lanes with a side longer than two bins; a step's rising side is the previous step's
-            // falling side, so one compare per step carries both
-            const uint64_t longn = __ballot(dn > (R)1.999);
-            const uint64_t tailm = longn | longp;
-            longp = longn;
-            if (tailm) {
-              // keeps this wave-uniform test a branch of its own (the lane test below then costs
-              // nothing on the common path)
-              __asm__ volatile("");
-              if ((tailm >> lane) & 1) {
-                const int nd = ic - prv.ic;
-                const int nu = (int)ceil(t_next) - 1 - ic;
-                if (nd > W || nu > W) sweepmask |= 1;
-                const int nd_ = min(nd, W), nu_ = min(nu, W);
-                auto put = [&](int b, R w) {
-                  const int slot = b - base;
-                  if (slot < 0 || slot >= W) {
-                    winmask |= 1;
-                    return;
-                  }
-                  R2 v = win[slot * 64 + lane];
-                  v.x = fmaT(pth, w, v.x);
-                  v.y = fmaT(plo, w, v.y);
-                  win[slot * 64 + lane] = v;
-                };
-                for (int j = 2; j < nd_; j++) put(ic - j, fmaT(-(fr + (R)j), idp, (R)1));
-                for (int j = 2; j < nu_; j++) put(ic + 1 + j, fmaT(fr - (R)(1 + j), idn, (R)1));
+            if constexpr (TAIL) {
+              // lanes with a side longer than two bins; a step's rising side is the previous
+              // step's falling side, so one compare per step carries both
+              const uint64_t longn = __ballot(dn > (R)1.999);
+              const uint64_t tailm = longn | longp;
+              longp = longn;
+              if (tailm) {
+                // keeps this wave-uniform test a branch of its own (the lane test below then
+                // costs nothing on the common path)
+                __asm__ volatile("");
+                if ((tailm >> lane) & 1) {
+                  const int nd = ic - prv.ic;
+                  const int nu = (int)ceil(t_next) - 1 - ic;
+                  if (nd > W || nu > W) sweepmask |= 1;
+                  const int nd_ = min(nd, W), nu_ = min(nu, W);
+                  auto put = [&](int b, R w) {
+                    const int slot = b - base;
+                    if (slot < 0 || slot >= W) {
+                      winmask |= 1;
+                      return;
+                    }
+                    R2 v = win[slot * 64 + lane];
+                    v.x = fmaT(pth, w, v.x);
+                    v.y = fmaT(plo, w, v.y);
+                    win[slot * 64 + lane] = v;
+                  };
+                  for (int j = 2; j < nd_; j++) put(ic - j, fmaT(-(fr + (R)j), idp, (R)1));
+                  for (int j = 2; j < nu_; j++) put(ic + 1 + j, fmaT(fr - (R)(1 + j), idn, (R)1));
+                }
               }
             }
             // the step before becomes the next step
             prv.t = t_next;
             prv.ic = (int)t_next;
             prv.id = idn;
-            dxc = (R)dxn;
-            dyc = (R)dyn;
-            dzc = (R)dzn;
+            diff(pn, &dxc, &dyc, &dzc);
             delc = (R)dln;
           };
           // the last step of the period wraps tau_next by +nT; it and an odd step run singly,
@@ -627,16 +667,25 @@ __global__ __launch_bounds__(256) void tonal_reduce_kernel(const TonalCaseDev* _
 
 // Reference AoS (X[4], N[6], V[6], PR[2] per element-step, surfaces concatenated, element-major
 // with nT steps each) -> position and attribute records; the products are formed in FP64.
-__global__ void tonal_pos_kernel(const float* __restrict__ X, int64_t n, int nT, double SB,
-                                 PosRec* __restrict__ pos) {
+static_assert(sizeof(PosRec) == 64 && sizeof(PosSolve) == 48 && offsetof(PosRec, jt) == offsetof(PosSolve, jt),
+              "position record layout");
+__global__ void tonal_pos_kernel(const float* __restrict__ X, int64_t n, TimeConst k, PosRec* __restrict__ pos) {
+  const double m2C2 = -2.0 * k.C2;
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
     const float* x = X + 4 * q;
+    const double px = (double)x[0] - k.Ox, py = (double)x[1] - k.Oy, pz = (double)x[2] - k.Oz;
     PosRec o;
+    o.mx = px * m2C2;
+    o.my = py * m2C2;
+    o.mz = pz * m2C2;
+    o.q = __builtin_fma(pz, pz, __builtin_fma(py, py, px * px)) * k.C2;
+    o.wp = __builtin_fma(pz, k.wz, __builtin_fma(py, k.wy, px * k.wx));
+    o.jt = (double)(int)(q % k.nT) * k.SB;
     o.x = x[0];
     o.y = x[1];
     o.z = x[2];
-    o.jt = (double)(int)(q % nT) * SB;
+    o.pad = 0.0f;
     pos[q] = o;
   }
 }
@@ -660,19 +709,20 @@ __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __re
     for (int i = 0; i < 4; i++) bad |= !isfinite(x[i]);
     AttrT<R> o;
     double Vn = 0.0, VV = 0.0;
+    const double cL = ds * D1;
     for (int i = 0; i < 3; i++) {
       const double Vi = v[i], Vdi = v[3 + i], ni = nn[i], ndi = nn[3 + i];
       o.v[i] = (R)(Vi * inv_a0);
       o.vd[i] = (R)(Vdi * inv_a0);
-      o.l[i] = (R)(P * ni);
-      o.ld[i] = (R)(dP * ni + P * ndi);
+      o.l[i] = (R)(cL * (P * ni));
+      o.ld[i] = (R)(cL * (dP * ni + P * ndi));
       Vn += Vi * ni;
       VV += Vi * Vi;
     }
     o.aM1 = (R)(VV * inv_a0 - a0);
-    o.aLM = (R)(P * Vn);
+    o.aLM = (R)(cL * (P * Vn));
     o.cT = (R)(rhoRef * Vn * ds * a0 * D1);
-    o.cL = (R)(ds * D1);
+    o.unused = (R)0;
     attr[q] = o;
   }
   if (bad) atomicOr(flags, kFlagNumeric);  // non-finite element input
@@ -690,8 +740,8 @@ __global__ void reset_counter_kernel(int* c, uint32_t* flags) {
   *flags = 0u;
 }
 
-// Tent of every (element, step) for one observer, from the same canonical observer time: the
-// top rising bin ic (stored as ic mod nT, with its period index floor(ic / nT)) and the two side
+// Tent of every (element, step) for one observer, from the same canonical observer time: the top
+// rising bin ic (stored as ic mod nT, with its period index floor(ic / nT)) and the two side
 // lengths.
 template <bool MOVING>
 __global__ void tonal_bins_kernel(const PosRec* __restrict__ pos, int64_t nE, TimeConst k,
@@ -701,10 +751,11 @@ __global__ void tonal_bins_kernel(const PosRec* __restrict__ pos, int64_t nE, Ti
   if (e >= nE) return;
   const int nT = k.nT;
   const PosRec* pe = pos + e * nT;
+  const LaneObs L = lane_obs(k, ox, oy, oz);
   auto tau = [&](int j) {
     const PosRec p = pe[j];
     double d;
-    return obs_time<MOVING>(k, p.jt, ox - p.x, oy - p.y, oz - p.z, &d);
+    return obs_time<MOVING, true>(L, p, &d);
   };
   double t_prev = tau(nT - 1) - (double)nT;
   double t_cur = tau(0);
@@ -743,32 +794,33 @@ __global__ void tonal_bins_kernel(const PosRec* __restrict__ pos, int64_t nE, Ti
   X(true, 16) X(true, 20) X(true, 24) X(true, 32) X(true, 48) X(true, 64) X(false, 16) X(false, 20) \
   X(false, 24) X(false, 32)
 
-hipError_t launch_tonal_eval(bool mixed, bool moving, int W, int grid, hipStream_t s,
+hipError_t launch_tonal_eval(bool mixed, bool moving, bool tail, int W, int grid, hipStream_t s,
                              const TonalCaseDev* cases, const Tile* tiles, const Item* items,
                              const int* qoff, int* counter) {
   dim3 b(64);
-#define ACUM_EVAL(M, WW)                                                                       \
-  if (mixed == M && W == WW) {                                                                 \
-    if (moving)                                                                                \
-      hipLaunchKernelGGL((tonal_eval_kernel<M, true, WW>), dim3(grid), b, 0, s, cases, tiles,  \
-                         items, qoff, counter);                                              \
-    else                                                                                       \
-      hipLaunchKernelGGL((tonal_eval_kernel<M, false, WW>), dim3(grid), b, 0, s, cases, tiles, \
-                         items, qoff, counter);                                              \
-    return hipGetLastError();                                                                  \
+#define ACUM_EVAL1(M, MV, T, WW)                                                                  \
+  if (mixed == M && moving == MV && tail == T && W == WW) {                                       \
+    hipLaunchKernelGGL((tonal_eval_kernel<M, MV, T, WW>), dim3(grid), b, 0, s, cases, tiles, items, \
+                       qoff, counter);                                                            \
+    return hipGetLastError();                                                                     \
   }
+#define ACUM_EVAL(M, WW) \
+  ACUM_EVAL1(M, true, true, WW) ACUM_EVAL1(M, true, false, WW) ACUM_EVAL1(M, false, true, WW) ACUM_EVAL1(M, false, false, WW)
   ACUM_EVAL_VARIANTS(ACUM_EVAL)
 #undef ACUM_EVAL
+#undef ACUM_EVAL1
   return hipErrorInvalidValue;
 }
 
+// Occupancy of the tail-free variant (the register budget is held to the window's occupancy, so
+// the tail does not change it).
 hipError_t eval_occupancy(bool mixed, bool moving, int W, int* blocks_per_cu) {
 #define ACUM_OCC(M, WW)                                                                        \
   if (mixed == M && W == WW)                                                                   \
     return moving ? hipOccupancyMaxActiveBlocksPerMultiprocessor(                              \
-                        blocks_per_cu, tonal_eval_kernel<M, true, WW>, 64, 0)                  \
+                        blocks_per_cu, tonal_eval_kernel<M, true, false, WW>, 64, 0)           \
                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(                              \
-                        blocks_per_cu, tonal_eval_kernel<M, false, WW>, 64, 0);
+                        blocks_per_cu, tonal_eval_kernel<M, false, false, WW>, 64, 0);
   ACUM_EVAL_VARIANTS(ACUM_OCC)
 #undef ACUM_OCC
   return hipErrorInvalidValue;
@@ -792,9 +844,9 @@ hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* ca
   return hipGetLastError();
 }
 
-hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, int nT, double SB, void* pos) {
+hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, const TimeConst& k, void* pos) {
   const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
-  hipLaunchKernelGGL(tonal_pos_kernel, dim3(grid), dim3(256), 0, s, X, n, nT, SB, (PosRec*)pos);
+  hipLaunchKernelGGL(tonal_pos_kernel, dim3(grid), dim3(256), 0, s, X, n, k, (PosRec*)pos);
   return hipGetLastError();
 }
 

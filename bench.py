@@ -189,6 +189,90 @@ def summarize_sq(d):
     return 0
 
 
+# Counter passes of the instruction-mix profiles (rocprofv3 takes at most 8 SQ counters and 2
+# GRBM counters per run).
+COUNTER_PASSES = {
+    "fp64": ("SQ_INSTS_VALU", "SQ_INSTS_MFMA", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64",
+             "SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_TRANS_F64", "SQ_INSTS_VALU_CVT", "SQ_INSTS_VALU_INT32"),
+    "fp32": ("SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_TRANS_F32",
+             "SQ_INSTS_VALU_INT64", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_SMEM"),
+    "cycles": ("SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_ANY",
+               "SQ_WAIT_INST_ANY", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_INSTS_VMEM", "GRBM_GUI_ACTIVE", "GRBM_COUNT"),
+}
+# kernels the instruction-mix profiles report, and the work one launch does (for per-unit rows)
+COUNTER_KERNELS = (
+    ("tonal_eval_kernel<true, true, 20>", "mixed eval (four-blade rotor)", 80000 * 1024 * 1024 / 64, "wave-eval"),
+    ("tonal_eval_kernel<false, true, 20>", "FP64 eval (four-blade rotor)", 80000 * 1024 * 1024 / 64, "wave-eval"),
+    ("bpm_obs_kernel<17>", "broadband observer kernel (4 x 60 sections, 360 steps, 1024 observers, 34 bands)",
+     240 * 360 * 1024 * 34 / 64, "wave band-eval"),
+)
+
+
+def summarize_counters(name, dirs):
+    """Per-launch averages of the counters of several rocprofv3 --pmc runs (one COUNTER_PASSES
+    group each) for the kernels in COUNTER_KERNELS, into profiles/<name>.md."""
+    import csv
+    per = {}  # kernel -> counter -> list of per-dispatch values
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            disp = {}
+            for r in csv.DictReader(open(f)):
+                k = r["Kernel_Name"].replace("(anonymous namespace)::", "").replace("acum::", "")
+                key = (int(r["Dispatch_Id"]), k, r["Counter_Name"])
+                disp[key] = disp.get(key, 0.0) + float(r["Counter_Value"])
+            for (_, k, c), v in disp.items():
+                per.setdefault(k, {}).setdefault(c, []).append(v)
+    lines = [f"# Instruction mix and issue counters: {name.replace('_', ' ')}", "",
+             "Commands (repository root `R`, output directory `OUT`, on the GPU host; one counter group per run):",
+             "", "```", "cd /tmp && export TMPDIR=/tmp"]
+    for g, cs in COUNTER_PASSES.items():
+        lines.append(f"rocprofv3 --pmc {' '.join(cs)} --output-format csv -d $OUT/{g} -o p -- \\")
+        lines.append("    python3 $R/bench.py --steps 1 --warmup 0 --no-bpm --no-batched --no-stress --oracle-obs 1")
+        lines.append(f"rocprofv3 --pmc {' '.join(cs)} --output-format csv -d $OUT/bpm_{g} -o p -- \\")
+        lines.append("    python3 $R/bench.py --workload bpm --steps 1")
+    lines += [f"python3 $R/bench.py --summarize-counters {name} $OUT/*", "```", "",
+              "Values are per launch (averaged over the launches of each run). SQ_WAVE_CYCLES, SQ_BUSY_CYCLES,",
+              "SQ_ACTIVE_INST_* and SQ_WAIT_INST_ANY count in quad-cycles (units of 4 cycles), summed over",
+              "waves (WAVE_CYCLES, ACTIVE, WAIT) or over the shader engines (BUSY). VALU issue share =",
+              "ACTIVE_INST_VALU / WAVE_CYCLES x waves per SIMD: the fraction of a SIMD's cycles its VALU",
+              "issues, with that many waves resident.", ""]
+    for kname, label, units, unit_name in COUNTER_KERNELS:
+        ks = [k for k in per if kname in k]
+        if not ks:
+            continue
+        cnt = {}
+        for k in ks:
+            for c, v in per[k].items():
+                cnt.setdefault(c, []).extend(v)
+        avg = {c: float(np.mean(v)) for c, v in cnt.items()}
+        lines += [f"## {label}: `{kname}`", "", f"| counter | per launch | per {unit_name} |", "|---|---|---|"]
+        for g in COUNTER_PASSES.values():
+            for c in g:
+                if c in avg:
+                    lines.append(f"| {c} | {avg[c]:.4g} | {avg[c] / units:.3f} |")
+        if "SQ_INSTS_VALU" in avg:
+            f64 = sum(avg.get(f"SQ_INSTS_VALU_{t}_F64", 0.0) for t in ("ADD", "MUL", "FMA", "TRANS"))
+            f32 = sum(avg.get(f"SQ_INSTS_VALU_{t}_F32", 0.0) for t in ("ADD", "MUL", "FMA", "TRANS"))
+            lines += ["", f"FP64 arithmetic {f64 / units:.2f}, FP32 arithmetic {f32 / units:.2f}, "
+                          f"conversions {avg.get('SQ_INSTS_VALU_CVT', 0) / units:.2f}, "
+                          f"INT32 {avg.get('SQ_INSTS_VALU_INT32', 0) / units:.2f}, "
+                          f"transcendental F64 {avg.get('SQ_INSTS_VALU_TRANS_F64', 0) / units:.2f} / "
+                          f"F32 {avg.get('SQ_INSTS_VALU_TRANS_F32', 0) / units:.2f}, "
+                          f"matrix-core {avg.get('SQ_INSTS_MFMA', 0) / units:.2f} instructions per {unit_name}."]
+        if "SQ_WAVE_CYCLES" in avg and "SQ_ACTIVE_INST_VALU" in avg:
+            lines += [f"VALU issue per wave cycle: {avg['SQ_ACTIVE_INST_VALU'] / avg['SQ_WAVE_CYCLES']:.3f} "
+                      f"(times the resident waves per SIMD gives the SIMD's VALU issue share); waits "
+                      f"{avg.get('SQ_WAIT_INST_ANY', 0) / avg['SQ_WAVE_CYCLES']:.3f} of wave cycles; "
+                      f"matrix-core busy {avg.get('SQ_VALU_MFMA_BUSY_CYCLES', 0) / max(avg.get('GRBM_GUI_ACTIVE', 1), 1):.3g} "
+                      f"per GPU cycle."]
+        lines.append("")
+    out = os.path.join(ROOT, "profiles", f"{name}.md")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+    return 0
+
+
 def summarize_stats(d, workload="rotor"):
     """Kernel time table of one rocprofv3 --kernel-trace --stats run of
     `bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2` (both tonal precisions), or with
@@ -510,6 +594,7 @@ def main():
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
     ap.add_argument("--summarize-sq", metavar="DIR")
     ap.add_argument("--summarize-stats", metavar="DIR")
+    ap.add_argument("--summarize-counters", nargs="+", metavar=("NAME", "DIR"))
     ap.add_argument("--workload", choices=("rotor", "batched", "stress", "bpm"), default="rotor",
                     help="rotor: the four-blade rotor line with the broadband, batched and stress legs "
                          "(default); batched: the uneven cases alone; stress: the coaxial case alone; "
@@ -523,6 +608,8 @@ def main():
         return summarize_pmc(*args.summarize_pmc)
     if args.summarize_sq:
         return summarize_sq(args.summarize_sq)
+    if args.summarize_counters:
+        return summarize_counters(args.summarize_counters[0], args.summarize_counters[1:])
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(observers would be sharded across GPUs with no communication); run with --gpus 1", file=sys.stderr)

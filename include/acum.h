@@ -34,7 +34,8 @@ enum {
   ACUM_ERR_IO = -2,      /* missing, truncated or malformed input file */
   ACUM_ERR_DEVICE = -3,  /* no GPU, or a HIP runtime call failed */
   ACUM_ERR_NOMEM = -4,   /* host or device allocation failed */
-  ACUM_ERR_NUMERIC = -5, /* retarded-time discriminant < 0, source on observer, |Mr| >= 1 */
+  ACUM_ERR_NUMERIC = -5, /* an observer within 1e-6 m of a source position, 1 - Mr <= 0 (or a
+                            non-increasing observer time), a non-finite result                 */
   ACUM_ERR_PLAN = -6     /* a tile's observer-time window exceeded its planned bound */
 };
 
@@ -166,7 +167,8 @@ int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t);
 /* Autotuner.  Each case of the staged batch falls in a shape bucket: precision, still air or
  * moving medium, the rounded-up base-2 logarithms of its element, step and observer counts, and
  * the device's compute units.  A bucket not yet in the context's cache is tuned on its case:
- * successive halving over window x spread x source-motion bound, then max_tile, each candidate
+ * successive halving over window (16, 20, 24) x spread (2, 3, 4) x source-motion bound (0.5,
+ * 0.6, 0.8 of the worst case), then max_tile (64, 256), each candidate
  * timed on the case's first 128 observers, keeping the best third per round with twice the
  * passes.  The chosen shape is cached by bucket, and every later plan of a case in that bucket
  * (alone or in any batch) uses it, so batched and single runs of a case stay bit-identical.

@@ -12,15 +12,22 @@
 //
 // Observer time and the tent are written in the canonical FP64 form shared with the GPU
 // kernels (same operations, same order, explicit fma, no other contraction), so bin indices
-// agree exactly.  Time is measured in observer-time bins (units of dPsi):
-//   d = obs - src; c = fma(dz,dz, fma(dy,dy, dx*dx))                      (core.cu:267)
-//   still air: s = sqrt(c), r = s, tau(j) = fma(s, (oM/a0)/dPsi, jt)
-//   otherwise: b = fma(dz,vz, fma(dy,vy, dx*vx)), s = sqrt(fma(b, b, c na)),
-//              t = (s - b) / na, r = a0 t, tau(j) = fma(t, oM/dPsi, jt)
-//   with na = a0^2 - |vinf|^2 and jt = j (dTau oM/dPsi): tau = (j dTau + r/a0) oM / dPsi
-//   (core.cu:156).  t is the root (-b' - sqrt(b'^2 - 4a'c))/(2a') acous_dist takes
-//   (core.cu:261-279), a' = |vinf|^2 - a0^2, b' = -2 d.vinf, positive for subsonic vinf; the
-//   form above drops the factors of 2 and the sign flips.
+// agree exactly.  Time is measured in observer-time bins (units of dPsi).  With KB = oM / dPsi,
+// na = a0^2 - |vinf|^2, C2 = KB (KB / na), w = vinf (KB / na) (zero in still air), SB = dTau KB,
+// aB = a0 / KB and the origin O = the centre of the bounding box of every source position:
+//   per source (step j):  p = src - O,  q = C2 fma(pz,pz, fma(py,py, px px)),  m = (-2 C2) p,
+//                         wp = fma(pz,wz, fma(py,wy, px wx)),  jt = j SB
+//   per observer:         o = obs - O,  qo = C2 fma(oz,oz, fma(oy,oy, ox ox)),
+//                         wo = fma(oz,wz, fma(oy,wy, ox wx))
+//   per eval:             cc = fma(ox,mx, fma(oy,my, fma(oz,mz, qo + q)))      = C2 |obs - src|^2
+//                         b = wo - wp;  T = sqrt(fma(b, b, cc)) - b;  tau = T + jt;  r = aB T
+// (still air: b = 0, T = sqrt(cc)).  T = KB t is the delay in bins, t the root
+// (-b' - sqrt(b'^2 - 4 a' c)) / (2 a') of a' t^2 + b' t + c = 0 with a' = |vinf|^2 - a0^2 < 0,
+// b' = -2 d.vinf, c = |d|^2 that acous_dist takes (core.cu:261-279): t = (s - d.vinf) / na with
+// s = sqrt((d.vinf)^2 + c na), scaled by KB inside the square root; tau = (j dTau + r/a0) oM/dPsi
+// (core.cu:156).  |obs - src|^2 is expanded about the origin so that each eval needs three fmas
+// and one add for it (the rounding error of the expansion is that of |o|^2 + |p|^2, which with
+// the origin at the sources' centre is about |obs - src|^2 for observers outside the sources).
 // findpts/bsearch (core.cu:290-355) pick the grid time at or below the shifted observer time
 // and the two sweeps (core.cu:196-231) visit the grid times in (ttem, ttemp] (weights rising
 // from 0 to 1) and in (ttemp, ttep) (weights falling from 1 to 0), wrapping periodically.  In
@@ -31,11 +38,13 @@
 // in exact arithmetic; tests check the two forms agree on sampled cases.
 //
 // Error codes follow the library's contract (include/acum.h), not the reference, which computes
-// on with whatever the arithmetic gives: a non-finite observer position is ACUM_ERR_ARG; an eval
-// with r <= 0 (source on an observer), 1 - Mr <= 0, observer time that does not increase from
+// on with whatever the arithmetic gives: a non-finite observer position is ACUM_ERR_ARG; an
+// observer within 1e-6 m of a source position (a source on an observer; |obs - src|^2 from the
+// parsed floats), an eval with r <= 0 or 1 - Mr <= 0, observer time that does not increase from
 // one step to the next, or a non-finite observer time or integrand is ACUM_ERR_NUMERIC.
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 #include <omp.h>
@@ -46,7 +55,9 @@ namespace {
 struct TimeConst {
   double a0, r2b, KB, SB;
   double vx, vy, vz;
-  double wx, wy, wz, C2, aB;  // moving medium: w = vinf KB / na, C2 = KB^2 / na, aB = a0 / KB
+  double wx, wy, wz, C2, aB;  // moving medium: w = vinf KB / na; C2 = KB^2 / na; aB = a0 / KB
+  double Ox, Oy, Oz;          // origin: centre of the sources' bounding box
+  double m2C2;                // -2 C2
   int64_t nT;
   bool still;  // vinf == 0
 };
@@ -71,31 +82,63 @@ TimeConst make_time_const(const acum_tonal_case* c) {
   k.wz = k.vz * kn;
   k.C2 = k.KB * kn;
   k.aB = k.a0 / k.KB;
+  k.m2C2 = -2.0 * k.C2;
   k.still = (k.vx == 0.0 && k.vy == 0.0 && k.vz == 0.0);
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t s = 0; s < c->nSurf; s++) {
+    const float* X = c->X + c->XPtr[s];
+    for (int64_t i = 0; i < c->E[s] * c->nTime; i++)
+      for (int d = 0; d < 3; d++) {
+        const double v = X[4 * i + d];
+        lo[d] = std::min(lo[d], v);
+        hi[d] = std::max(hi[d], v);
+      }
+  }
+  k.Ox = (lo[0] + hi[0]) * 0.5;
+  k.Oy = (lo[1] + hi[1]) * 0.5;
+  k.Oz = (lo[2] + hi[2]) * 0.5;
   return k;
 }
 
-// Observer time of step j in bins and the retarded distance r (acous_dist, core.cu:261-279;
-// ttemp, core.cu:156).  Moving medium: T = KB t, the delay in bins, is the positive root of
-// |d - vinf T / KB| = a0 T / KB, i.e. T = sqrt(b^2 + c KB^2 / na) - b with b = KB (d.vinf) / na
-// and na = a0^2 - |vinf|^2; tau = T + j SB and r = a0 T / KB.
-inline double obs_tau_r(const TimeConst& k, int64_t j, double ox, double oy, double oz,
-                        const float* src, double* r) {
-  const double dx = ox - (double)src[0];
-  const double dy = oy - (double)src[1];
-  const double dz = oz - (double)src[2];
-  const double c = std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+// Per-observer constants of the canonical observer time.
+struct ObsConst {
+  double ox, oy, oz, qo, wo;
+};
+
+inline ObsConst obs_const(const TimeConst& k, double obx, double oby, double obz) {
+  ObsConst o;
+  o.ox = obx - k.Ox;
+  o.oy = oby - k.Oy;
+  o.oz = obz - k.Oz;
+  o.qo = std::fma(o.oz, o.oz, std::fma(o.oy, o.oy, o.ox * o.ox)) * k.C2;
+  o.wo = std::fma(o.oz, k.wz, std::fma(o.oy, k.wy, o.ox * k.wx));
+  return o;
+}
+
+// Observer time of step j in bins and the retarded distance r in metres (acous_dist,
+// core.cu:261-279; ttemp, core.cu:156), in the canonical form above.
+inline double obs_tau_r(const TimeConst& k, int64_t j, const ObsConst& o, const float* src, double* r) {
+  const double px = (double)src[0] - k.Ox;
+  const double py = (double)src[1] - k.Oy;
+  const double pz = (double)src[2] - k.Oz;
+  const double q = std::fma(pz, pz, std::fma(py, py, px * px)) * k.C2;
+  const double mx = px * k.m2C2, my = py * k.m2C2, mz = pz * k.m2C2;
+  const double wp = std::fma(pz, k.wz, std::fma(py, k.wy, px * k.wx));
   const double jt = (double)j * k.SB;
+  const double cc = std::fma(o.ox, mx, std::fma(o.oy, my, std::fma(o.oz, mz, o.qo + q)));
+  double T;
   if (k.still) {
-    const double s = std::sqrt(c);
-    *r = s;
-    return std::fma(s, k.r2b, jt);
+    T = std::sqrt(cc);
+  } else {
+    const double b = o.wo - wp;
+    T = std::sqrt(std::fma(b, b, cc)) - b;
   }
-  const double b = std::fma(dz, k.wz, std::fma(dy, k.wy, dx * k.wx));
-  const double T = std::sqrt(std::fma(b, b, c * k.C2)) - b;
   *r = T * k.aB;
   return T + jt;
 }
+
+// an observer this close (metres) to a source position is a source on an observer
+constexpr double kCoincident = 1e-6;
 
 struct Eval {
   double pT, pL;
@@ -200,6 +243,7 @@ int tonal_impl(const acum_tonal_case* c, Out* pT, Out* pL, Out* pA, int nthreads
       std::fill(bL.begin(), bL.end(), (R)0);
       std::fill(bA.begin(), bA.end(), (R)0);
       const double ox = c->rObs[3 * o], oy = c->rObs[3 * o + 1], oz = c->rObs[3 * o + 2];
+      const ObsConst oc = obs_const(k, ox, oy, oz);
       bool ok = true;
       for (int64_t s = 0; s < c->nSurf && ok; s++) {
         for (int64_t j = 0; j < c->E[s] && ok; j++) {
@@ -209,7 +253,13 @@ int tonal_impl(const acum_tonal_case* c, Out* pT, Out* pL, Out* pA, int nthreads
           const float* Pe = c->PR + c->PRPtr[s] + j * nT * 2;
           // One retarded solve per (element, step); the reference solves three times
           // (core.cu:84,189,193) with identical operands, hence identical values.
-          for (int64_t t = 0; t < nT; t++) tt[t] = obs_tau_r(k, t, ox, oy, oz, Xe + 4 * t, &rr[t]);
+          for (int64_t t = 0; t < nT; t++) {
+            tt[t] = obs_tau_r(k, t, oc, Xe + 4 * t, &rr[t]);
+            // a source on the observer: within kCoincident of it, |obs - src|^2 from the floats
+            const double dx = ox - (double)Xe[4 * t], dy = oy - (double)Xe[4 * t + 1],
+                         dz = oz - (double)Xe[4 * t + 2];
+            if (std::fma(dz, dz, std::fma(dy, dy, dx * dx)) < kCoincident * kCoincident) ok = false;
+          }
           for (int64_t t = 0; t < nT && ok; t++) {
             const int64_t mm = (t > 0) ? t - 1 : nT - 1;       // core.cu:173-179
             const int64_t mp = (t < nT - 1) ? t + 1 : 0;       // core.cu:180-186
@@ -265,13 +315,13 @@ int orc_tonal_bins(const acum_tonal_case* c, int64_t iobs, int32_t* ipts, int32_
   if (!c || iobs < 0 || iobs >= c->nObs) return ACUM_ERR_ARG;
   const TimeConst k = make_time_const(c);
   const int64_t nT = c->nTime;
-  const double ox = c->rObs[3 * iobs], oy = c->rObs[3 * iobs + 1], oz = c->rObs[3 * iobs + 2];
+  const ObsConst oc = obs_const(k, c->rObs[3 * iobs], c->rObs[3 * iobs + 1], c->rObs[3 * iobs + 2]);
   std::vector<double> rr(nT), tt(nT);
   int64_t idx = 0;
   for (int64_t s = 0; s < c->nSurf; s++) {
     for (int64_t j = 0; j < c->E[s]; j++) {
       const float* Xe = c->X + c->XPtr[s] + j * nT * 4;
-      for (int64_t t = 0; t < nT; t++) tt[t] = obs_tau_r(k, t, ox, oy, oz, Xe + 4 * t, &rr[t]);
+      for (int64_t t = 0; t < nT; t++) tt[t] = obs_tau_r(k, t, oc, Xe + 4 * t, &rr[t]);
       for (int64_t t = 0; t < nT; t++, idx++) {
         const int64_t mm = (t > 0) ? t - 1 : nT - 1;
         const int64_t mp = (t < nT - 1) ? t + 1 : 0;
@@ -301,13 +351,13 @@ int orc_tonal_bins(const acum_tonal_case* c, int64_t iobs, int32_t* ipts, int32_
 double orc_obs_tau(const acum_tonal_case* c, const float* src, const float* obs, int64_t j) {
   const TimeConst k = make_time_const(c);
   double r;
-  return obs_tau_r(k, j, obs[0], obs[1], obs[2], src, &r);
+  return obs_tau_r(k, j, obs_const(k, obs[0], obs[1], obs[2]), src, &r);
 }
 
 double orc_retarded_r(const acum_tonal_case* c, const float* src, const float* obs) {
   const TimeConst k = make_time_const(c);
   double r;
-  obs_tau_r(k, 0, obs[0], obs[1], obs[2], src, &r);
+  obs_tau_r(k, 0, obs_const(k, obs[0], obs[1], obs[2]), src, &r);
   return r;
 }
 

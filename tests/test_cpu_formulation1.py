@@ -123,7 +123,7 @@ def errors(nT):
     return O.rel_l2(fT, oT), O.rel_l2(fL, oL)
 
 
-def test_formulation1_agrees_with_1a_to_second_order():
+def test_formulation1_agrees_with_1a_as_the_step_falls():
     eT1, eL1 = errors(360)
     eT2, eL2 = errors(720)
     # agreement to discretisation error that falls at least twice per halving of the step
