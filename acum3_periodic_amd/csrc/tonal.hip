@@ -94,11 +94,11 @@ __device__ __forceinline__ double obs_time(const LaneObs& L, const P& p, double*
   const double cc = __builtin_fma(L.ox, p.mx, __builtin_fma(L.oy, p.my, __builtin_fma(L.oz, p.mz, L.qo + p.q)));
   double T;
   if (!MOVING) {
-    T = EXACT ? sqrt_pos(cc) : sqrt_pos1(cc);
+    T = EXACT ? sqrt_pos(cc) : sqrt_pos_fast(cc);
   } else {
     const double b = L.wo - p.wp;
     const double x = __builtin_fma(b, b, cc);
-    T = (EXACT ? sqrt_pos(x) : sqrt_pos1(x)) - b;
+    T = (EXACT ? sqrt_pos(x) : sqrt_pos_fast(x)) - b;
   }
   *delay = T;
   return T + p.jt;
