@@ -492,27 +492,31 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             diff(pn, &dxc, &dyc, &dzc);
             delc = (R)dln;
           };
-          // the last step of the period wraps tau_next by +nT; it and an odd step run singly,
-          // each followed by a swap of the buffers (once or twice per pass)
+          // Steps run in pairs with the state and record buffers in alternating roles (A is the
+          // first step's current state, B its next), so nothing is copied between steps.  An odd
+          // step before the period's last runs first, singly, and the pairs then start with the
+          // roles swapped; the period's last step (which wraps tau_next by +nT) runs singly after
+          // them.  The pass's last footprint is then checked.
           const int kstop = (kend == nT) ? nT - 1 : kend;
-          int kk = k0;
-          for (; kk + 1 < kstop; kk += 2) {
-            step(kk, false, s0, s1, p0, p1, a0r, a1r);
-            step(kk + 1, false, s1, s0, p1, p0, a1r, a0r);
+          auto pairs = [&](int kk, Step& A, Step& B, PosRec& pA, PosRec& pB, Rec& rA, Rec& rB)
+                           __attribute__((always_inline)) {
+            for (; kk + 1 < kstop; kk += 2) {
+              step(kk, false, A, B, pA, pB, rA, rB);
+              step(kk + 1, false, B, A, pB, pA, rB, rA);
+            }
+            int last_ic = B.ic;  // the current state of the last step evaluated
+            if (kk < kend) {
+              step(kk, true, A, B, pA, pB, rA, rB);
+              last_ic = A.ic;
+            }
+            winmask |= __ballot(last_ic - 1 - base > W - 4);
+          };
+          if ((kstop - k0) & 1) {
+            step(k0, false, s0, s1, p0, p1, a0r, a1r);
+            pairs(k0 + 1, s1, s0, p1, p0, a1r, a0r);
+          } else {
+            pairs(k0, s0, s1, p0, p1, a0r, a1r);
           }
-          for (; kk < kend; kk++) {
-            step(kk, kk == nT - 1, s0, s1, p0, p1, a0r, a1r);
-            const Step ts = s0;
-            s0 = s1;
-            s1 = ts;
-            const PosRec tp = p0;
-            p0 = p1;
-            p1 = tp;
-            const Rec ta = a0r;
-            a0r = a1r;
-            a1r = ta;
-          }
-          winmask |= __ballot(s1.ic - 1 - base > W - 4);  // the pass's last footprint
         }
         // add the window into this item's slab row and clear it (untouched slots hold zeros)
         int b = base % nT;

@@ -201,8 +201,8 @@ COUNTER_PASSES = {
 }
 # kernels the instruction-mix profiles report, and the work one launch does (for per-unit rows)
 COUNTER_KERNELS = (
-    ("tonal_eval_kernel<true, true, 20>", "mixed eval (four-blade rotor)", 80000 * 1024 * 1024 / 64, "wave-eval"),
-    ("tonal_eval_kernel<false, true, 20>", "FP64 eval (four-blade rotor)", 80000 * 1024 * 1024 / 64, "wave-eval"),
+    ("tonal_eval_kernel<true, true, false, 20>", "mixed eval (four-blade rotor)", 80000 * 1024 * 1024 / 64, "wave-eval"),
+    ("tonal_eval_kernel<false, true, false, 20>", "FP64 eval (four-blade rotor)", 80000 * 1024 * 1024 / 64, "wave-eval"),
     ("bpm_obs_kernel<17>", "broadband observer kernel (4 x 60 sections, 360 steps, 1024 observers, 34 bands)",
      240 * 360 * 1024 * 34 / 64, "wave band-eval"),
 )
