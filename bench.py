@@ -27,14 +27,14 @@ more than 0.01 dB off.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--oracle-obs N] [--no-fp64] [--no-bpm]
                     [--no-batched] [--no-stress] [--cases N --wave M]
     python bench.py --workload batched|stress|bpm          # one leg alone (batched: --cases of 64)
-    python bench.py --summarize-pmc FETCH_DIR WRITE_DIR   # HBM traffic summary into profiles/
-    python bench.py --summarize-sq DIR                     # instruction mix into profiles/
-    python bench.py --summarize-stats DIR                  # kernel times into profiles/
+    python bench.py --summarize-pmc FETCH_DIR WRITE_DIR      # HBM traffic summary into profiles/
+    python bench.py [--workload bpm] --summarize-counters DIR...  # instruction mix into profiles/
+    python bench.py [--workload bpm] --summarize-stats DIR     # kernel times into profiles/
 
 Roofline of the eval kernel: 100 FLOP per eval (70 FP32 + 30 FP64 in the mixed path) against
 the MI355X vector peaks (it is bound by vector-ALU issue); `traffic` is the kernel's measured HBM
 bytes per launch, read from the committed rocprofv3 counter summary
-(profiles/tonal_eval_c2_hbm_traffic.md) when present.
+(profiles/tonal_eval_hbm_traffic_rotor_4x20000_elements_1024_steps_1024_observers.md) when present.
 """
 import argparse
 import ctypes as C
@@ -58,7 +58,12 @@ HBM_PEAK = 8.0e12                           # B/s
 # ISO 1/3-octave band centres, 10 Hz - 20 kHz (the bands of the tonal spectra leg)
 ISO_BANDS = (10, 12.5, 16, 20, 25, 31.5, 40, 50, 63, 80, 100, 125, 160, 200, 250, 315, 400, 500, 630, 800,
              1000, 1250, 1600, 2000, 2500, 3150, 4000, 5000, 6300, 8000, 10000, 12500, 16000, 20000)
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "tonal_eval_c2_hbm_traffic.md")
+# workloads the committed profiles cover, as they appear in the profiles' file names
+TONAL_PROFILE = "rotor_4x20000_elements_1024_steps_1024_observers"
+BPM_PROFILE = "4x60_sections_360_steps_1024_observers_34_bands"
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", f"tonal_eval_hbm_traffic_{TONAL_PROFILE}.md")
+# the headline leg alone (the profiled runs leave the other legs out)
+TONAL_ONLY = "--no-bpm --no-batched --no-stress"
 EVAL_KERNEL = "tonal_eval_kernel"
 CALIB_BYTES = 1 << 30
 
@@ -96,8 +101,9 @@ def read_traffic():
 
 def summarize_pmc(fetch_dir, write_dir):
     """Turns two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) of
-    `bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1 --copy-calibration`
-    into the traffic summary, scaling each counter by its reading for the calibration copy."""
+    `bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --no-batched --no-stress --oracle-obs 1
+    --copy-calibration` into the traffic summary, scaling each counter by its reading for the
+    calibration copy."""
     import csv
 
     def load(d, counter):
@@ -133,9 +139,9 @@ def summarize_pmc(fetch_dir, write_dir):
         "```",
         "cd /tmp && export TMPDIR=/tmp",
         "rocprofv3 --pmc FETCH_SIZE --output-format csv -d $OUT/pmcF -o p -- \\",
-        "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1 --copy-calibration",
+        f"    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 {TONAL_ONLY} --oracle-obs 1 --copy-calibration",
         "rocprofv3 --pmc WRITE_SIZE --output-format csv -d $OUT/pmcW -o p -- \\",
-        "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1 --copy-calibration",
+        f"    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 {TONAL_ONLY} --oracle-obs 1 --copy-calibration",
         "python3 $R/bench.py --summarize-pmc $OUT/pmcF $OUT/pmcW",
         "```",
         "",
@@ -155,40 +161,6 @@ def summarize_pmc(fetch_dir, write_dir):
     return 0
 
 
-SQ_COUNTERS = ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_SMEM", "SQ_INSTS_LDS",
-               "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_WAIT_INST_ANY")
-
-
-def summarize_sq(d):
-    """Per wave-eval instruction counts of the eval kernel from one rocprofv3 --pmc run with the
-    SQ_COUNTERS over `bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1`."""
-    import csv
-    tot = {}
-    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-        for r in csv.DictReader(open(f)):
-            if EVAL_KERNEL in r["Kernel_Name"]:
-                tot[r["Counter_Name"]] = tot.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
-    wave_evals = 80000 * 1024 * 1024 / 64  # c2: elements x steps x observers / 64 lanes
-    lines = ["# Instruction mix of the tonal eval kernel (mixed path): four-blade rotor, 4 x 20,000",
-             "# elements, 1024 steps, 1024 observers, one launch", "",
-             "Command (repository root `R`, output directory `OUT`, on the GPU host):", "", "```",
-             "cd /tmp && export TMPDIR=/tmp",
-             "rocprofv3 --pmc " + " ".join(SQ_COUNTERS) + " --output-format csv -d $OUT/pmcSQ -o p -- \\",
-             "    python3 $R/bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --oracle-obs 1",
-             "python3 $R/bench.py --summarize-sq $OUT/pmcSQ", "```", "",
-             f"Per wave-eval = counter / {wave_evals:.4g} (8.39e10 evals / 64 lanes). WAVE_CYCLES and",
-             "WAIT_INST_ANY count in units of 4 cycles.", "",
-             "| counter | total | per wave-eval |", "|---|---|---|"]
-    for k in SQ_COUNTERS:
-        if k in tot:
-            lines.append(f"| {k} | {tot[k]:.4g} | {tot[k] / wave_evals:.2f} |")
-    out = os.path.join(ROOT, "profiles", "tonal_eval_c2_sq_counters.md")
-    with open(out, "w") as f:
-        f.write("\n".join(lines) + "\n")
-    print("\n".join(lines))
-    return 0
-
-
 # Counter passes of the instruction-mix profiles (rocprofv3 takes at most 8 SQ counters and 2
 # GRBM counters per run).
 COUNTER_PASSES = {
@@ -200,17 +172,17 @@ COUNTER_PASSES = {
                "SQ_WAIT_INST_ANY", "SQ_VALU_MFMA_BUSY_CYCLES", "SQ_INSTS_VMEM", "GRBM_GUI_ACTIVE", "GRBM_COUNT"),
 }
 # kernels the instruction-mix profiles report, and the work one launch does (for per-unit rows)
-COUNTER_KERNELS = (
-    ("tonal_eval_kernel<true, true, false, 20>", "mixed eval (four-blade rotor)", 80000 * 1024 * 1024 / 64, "wave-eval"),
-    ("tonal_eval_kernel<false, true, false, 20>", "FP64 eval (four-blade rotor)", 80000 * 1024 * 1024 / 64, "wave-eval"),
-    ("bpm_obs_kernel<17>", "broadband observer kernel (4 x 60 sections, 360 steps, 1024 observers, 34 bands)",
-     240 * 360 * 1024 * 34 / 64, "wave band-eval"),
-)
+COUNTER_KERNELS = {
+    "rotor": (("tonal_eval_kernel<true, true, false, 20>", "mixed eval", 80000 * 1024 * 1024 / 64, "wave-eval"),
+              ("tonal_eval_kernel<false, true, false, 20>", "FP64 eval", 80000 * 1024 * 1024 / 64, "wave-eval")),
+    "bpm": (("bpm_obs_kernel<17>", "broadband observer kernel", 240 * 360 * 1024 * 34 / 64, "wave band-eval"),),
+}
 
 
-def summarize_counters(name, dirs):
-    """Per-launch averages of the counters of several rocprofv3 --pmc runs (one COUNTER_PASSES
-    group each) for the kernels in COUNTER_KERNELS, into profiles/<name>.md."""
+def summarize_counters(dirs, workload="rotor"):
+    """Per-launch averages of the counters of the rocprofv3 --pmc runs (one COUNTER_PASSES group
+    each) under `dirs`, per unit of work, for the kernels of the workload (the four-blade rotor's
+    two eval kernels, or the broadband observer kernel), into profiles/."""
     import csv
     per = {}  # kernel -> counter -> list of per-dispatch values
     for d in dirs:
@@ -222,21 +194,32 @@ def summarize_counters(name, dirs):
                 disp[key] = disp.get(key, 0.0) + float(r["Counter_Value"])
             for (_, k, c), v in disp.items():
                 per.setdefault(k, {}).setdefault(c, []).append(v)
-    lines = [f"# Instruction mix and issue counters: {name.replace('_', ' ')}", "",
+    if workload == "bpm":
+        title = ("broadband observer kernel, 4 x 60 sections, 360 steps, 1024 observers, 34 bands "
+                 "(one launch per call)")
+        run = "python3 $R/bench.py --workload bpm --steps 1"
+        out = os.path.join(ROOT, "profiles", f"bpm_obs_instruction_mix_{BPM_PROFILE}.md")
+        summ = "python3 $R/bench.py --workload bpm --summarize-counters $OUT/*"
+    else:
+        title = ("tonal eval kernels, four-blade rotor (4 x 20,000 elements, 1024 steps, 1024 observers), "
+                 "mixed and FP64 paths, one launch each")
+        run = f"python3 $R/bench.py --steps 1 --warmup 0 {TONAL_ONLY} --oracle-obs 1"
+        out = os.path.join(ROOT, "profiles", f"tonal_eval_instruction_mix_{TONAL_PROFILE}.md")
+        summ = "python3 $R/bench.py --summarize-counters $OUT/*"
+    lines = [f"# Instruction mix and issue counters: {title}", "",
              "Commands (repository root `R`, output directory `OUT`, on the GPU host; one counter group per run):",
              "", "```", "cd /tmp && export TMPDIR=/tmp"]
     for g, cs in COUNTER_PASSES.items():
         lines.append(f"rocprofv3 --pmc {' '.join(cs)} --output-format csv -d $OUT/{g} -o p -- \\")
-        lines.append("    python3 $R/bench.py --steps 1 --warmup 0 --no-bpm --no-batched --no-stress --oracle-obs 1")
-        lines.append(f"rocprofv3 --pmc {' '.join(cs)} --output-format csv -d $OUT/bpm_{g} -o p -- \\")
-        lines.append("    python3 $R/bench.py --workload bpm --steps 1")
-    lines += [f"python3 $R/bench.py --summarize-counters {name} $OUT/*", "```", "",
-              "Values are per launch (averaged over the launches of each run). SQ_WAVE_CYCLES, SQ_BUSY_CYCLES,",
-              "SQ_ACTIVE_INST_* and SQ_WAIT_INST_ANY count in quad-cycles (units of 4 cycles), summed over",
-              "waves (WAVE_CYCLES, ACTIVE, WAIT) or over the shader engines (BUSY). VALU issue share =",
-              "ACTIVE_INST_VALU / WAVE_CYCLES x waves per SIMD: the fraction of a SIMD's cycles its VALU",
-              "issues, with that many waves resident.", ""]
-    for kname, label, units, unit_name in COUNTER_KERNELS:
+        lines.append(f"    {run}")
+    lines += [summ, "```", "",
+              "Values are per launch (averaged over the launches of each run) and per unit of work.",
+              "SQ_WAVE_CYCLES, SQ_BUSY_CYCLES, SQ_ACTIVE_INST_* and SQ_WAIT_INST_ANY count in quad-cycles",
+              "(units of 4 cycles), summed over waves (WAVE_CYCLES, ACTIVE, WAIT) or over the shader engines",
+              "(BUSY); GRBM_GUI_ACTIVE is summed over the 8 XCDs. VALU issue share = ACTIVE_INST_VALU /",
+              "WAVE_CYCLES x waves per SIMD: the fraction of a SIMD's cycles in which its VALU (or, two",
+              "quad-cycles per instruction, its matrix core) issues.", ""]
+    for kname, label, units, unit_name in COUNTER_KERNELS[workload]:
         ks = [k for k in per if kname in k]
         if not ks:
             continue
@@ -266,7 +249,6 @@ def summarize_counters(name, dirs):
                       f"matrix-core busy {avg.get('SQ_VALU_MFMA_BUSY_CYCLES', 0) / max(avg.get('GRBM_GUI_ACTIVE', 1), 1):.3g} "
                       f"per GPU cycle."]
         lines.append("")
-    out = os.path.join(ROOT, "profiles", f"{name}.md")
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -275,7 +257,8 @@ def summarize_counters(name, dirs):
 
 def summarize_stats(d, workload="rotor"):
     """Kernel time table of one rocprofv3 --kernel-trace --stats run of
-    `bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2` (both tonal precisions), or with
+    `bench.py --steps 3 --warmup 1 --no-bpm --no-batched --no-stress --oracle-obs 2` (both tonal
+    precisions), or with
     workload "bpm" of `bench.py --workload bpm --steps 3`."""
     import csv
     files = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)
@@ -288,14 +271,14 @@ def summarize_stats(d, workload="rotor"):
         cmd = ["    python3 $R/bench.py --workload bpm --steps 3",
                "python3 $R/bench.py --workload bpm --summarize-stats $OUT/prof"]
         note_ = ["`bpm_obs_kernel<17>` covers the 34 bands in two passes of 17 within one launch per call.", ""]
-        out = os.path.join(ROOT, "profiles", "bpm_c3_kernel_stats.md")
+        out = os.path.join(ROOT, "profiles", f"bpm_kernel_stats_{BPM_PROFILE}.md")
     else:
         head = ["# Kernel statistics: tonal path, four-blade rotor (4 x 20,000 elements, 1024 steps,",
                 "# 1024 observers), mixed and FP64 passes"]
-        cmd = ["    python3 $R/bench.py --steps 3 --warmup 1 --no-bpm --oracle-obs 2",
+        cmd = [f"    python3 $R/bench.py --steps 3 --warmup 1 {TONAL_ONLY} --oracle-obs 2",
                "python3 $R/bench.py --summarize-stats $OUT/prof"]
         note_ = ["`tonal_eval_kernel<true, ...>` is the mixed path, `<false, ...>` the FP64 path.", ""]
-        out = os.path.join(ROOT, "profiles", "tonal_c2_kernel_stats.md")
+        out = os.path.join(ROOT, "profiles", f"tonal_kernel_stats_{TONAL_PROFILE}.md")
     lines = head + ["", "Command (repository root `R`, output directory `OUT`, on the GPU host):", "", "```",
                     "cd /tmp && export TMPDIR=/tmp",
                     "rocprofv3 --kernel-trace --stats --output-format csv -d $OUT/prof -o bench -- \\"] + \
@@ -592,9 +575,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy first (for scaling HBM counters)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
-    ap.add_argument("--summarize-sq", metavar="DIR")
     ap.add_argument("--summarize-stats", metavar="DIR")
-    ap.add_argument("--summarize-counters", nargs="+", metavar=("NAME", "DIR"))
+    ap.add_argument("--summarize-counters", nargs="+", metavar="DIR")
     ap.add_argument("--workload", choices=("rotor", "batched", "stress", "bpm"), default="rotor",
                     help="rotor: the four-blade rotor line with the broadband, batched and stress legs "
                          "(default); batched: the uneven cases alone; stress: the coaxial case alone; "
@@ -606,10 +588,8 @@ def main():
         return summarize_stats(args.summarize_stats, args.workload)
     if args.summarize_pmc:
         return summarize_pmc(*args.summarize_pmc)
-    if args.summarize_sq:
-        return summarize_sq(args.summarize_sq)
     if args.summarize_counters:
-        return summarize_counters(args.summarize_counters[0], args.summarize_counters[1:])
+        return summarize_counters(args.summarize_counters, "bpm" if args.workload == "bpm" else "rotor")
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(observers would be sharded across GPUs with no communication); run with --gpus 1", file=sys.stderr)
