@@ -1133,9 +1133,14 @@ acum_tune default_tune(int prec) {
 // own and freed here).  Round 0 times every window x spread x source-motion candidate (with the
 // context's tile length and other limits) for one pass; the best quarter goes on, joined by
 // each survivor with the other tile length, for two passes; the best quarter of those for four
-// passes, and the fastest of them wins.  Windows above 24 bins are left out: their LDS halves
-// the kernel's occupancy, which has never paid.
-constexpr int kTuneObs = 128;
+// passes, and the fastest of them wins.  A plan's first pass is timed like the others (its
+// kernels are loaded and its buffers allocated before the timed events).  Windows above 24 bins
+// are left out: their LDS halves the kernel's occupancy, which has never paid.
+// The view's size sets how well its timings rank shapes for the whole case: on the four-blade
+// rotor and four of the seeded uneven cases, shapes tuned on 512 observers ran 1.5-6.7% faster
+// than the default shape on every case, while 128 and 256 observers picked shapes up to 2.7% and
+// 3.4% slower on some (their work items are too small to show the whole case's launch tail).
+constexpr int kTuneObs = 512;
 
 int tune_case(acum_ctx* ctx, const StagedCase& sc, int prec, TuneEntry* out, int* passes) {
   const bool mixed = prec == ACUM_MIXED;
@@ -1166,13 +1171,13 @@ int tune_case(acum_ctx* ctx, const StagedCase& sc, int prec, TuneEntry* out, int
     v.force = c.t;
     v.motion_level = 0;
     for (auto& p : view.plan) free_plan(p);
-    int rc = run_stage(ctx, view, prec, 1, false);  // plan and warm
-    if (rc == ACUM_OK) rc = run_stage(ctx, view, prec, n, false);
+    int rc = run_stage(ctx, view, prec, n, false);
     c.ms = rc == ACUM_OK ? ctx->timing.total_ms : 1e300;
     *passes += n;
     return rc == ACUM_ERR_PLAN ? ACUM_OK : rc;  // a shape whose windows cannot hold the case loses
   };
-  int rc = ACUM_OK;
+  // one untimed pass first, so no candidate's time holds the loading of the kernels
+  int rc = time_one(cand[0], 1);
   for (int round = 0, n = 1; round < 3 && rc == ACUM_OK; round++, n *= 2) {
     for (auto& c : cand)
       if ((rc = time_one(c, n)) != ACUM_OK) break;

@@ -169,9 +169,10 @@ int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t);
  * the device's compute units.  A bucket not yet in the context's cache is tuned on its case:
  * successive halving over window (16, 20, 24) x spread (2, 3, 4) x source-motion bound (0.5,
  * 0.6, 0.8 of the worst case), then max_tile (64, 256), each candidate
- * timed on the case's first 128 observers, keeping the best third per round with twice the
- * passes.  The chosen shape is cached by bucket, and every later plan of a case in that bucket
- * (alone or in any batch) uses it, so batched and single runs of a case stay bit-identical.
+ * timed on the case's first 512 observers (all of them in a smaller case), keeping the best
+ * quarter per round with twice the passes.  The chosen shape is cached by bucket, and every
+ * later plan of a case in that bucket (alone or in any batch) uses it, so batched and single
+ * runs of a case stay bit-identical.
  * best / best_ms: the shape and time per pass (on the observer subset) of case 0's bucket.
  * acum_get_timing then reports tune_cases, tune_hits and tune_passes of the call.             */
 int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* best, double* best_ms);
