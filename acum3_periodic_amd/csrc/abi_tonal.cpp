@@ -56,8 +56,9 @@ struct StagedCase {
   PosRec* d_pos = nullptr;                                  // shared by both precisions
   void* d_attr[2] = {nullptr, nullptr};                     // [ACUM_FP64], [ACUM_MIXED]
   float* d_out[3] = {nullptr, nullptr, nullptr};
-  int motion_level = 0;  // 0: the tuned source-motion bound; 1: at least 0.8; 2: the full bound
-                         // (raised by one after each window overflow)
+  int motion_level = 0;  // 0: the tuned source-motion bound; 1, 2: at least 0.6, 0.8 of the
+                         // worst case; 3: the full bound (raised after a window overflow to the
+                         // next level that widens the bound)
   bool forced = false;   // plan with `force` (the autotuner's candidates) instead of the
   acum_tune force{};     // context's shape for this case's bucket
 };
@@ -65,6 +66,18 @@ struct StagedCase {
 // Default of acum_tune.motion_permille: on the four-blade rotor a bound of 0.6 holds for every
 // tile and observer (0.45 overflows) and lengthens the passes from ~5 to ~6 steps.
 constexpr int kMotionPermille = 600;
+
+// The source-motion bound (per mille of the worst case) a case plans with at a motion level.
+int motion_permille_at(const acum_tune& t, int level) {
+  const int pm = t.motion_permille > 0 ? t.motion_permille : kMotionPermille;
+  switch (level) {
+    case 0: return pm;
+    case 1: return std::max(pm, 600);
+    case 2: return std::max(pm, 800);
+    default: return 1000;
+  }
+}
+constexpr int kMotionLevels = 4;
 
 // One eval launch: the work items of the cases that use one kernel variant (still air or moving
 // medium, window size), drained through kQueues queues; queue q holds items
@@ -653,8 +666,7 @@ void element_steps(StagedCase& sc) {
 int plan_tiles(acum_ctx* ctx, StagedCase& sc, const acum_tune& t, std::vector<Tile>& tiles) {
   const int64_t nT = sc.nT;
   // bins of observer time per metre of source motion: the full bound, or the reduced one
-  const int pm = t.motion_permille > 0 ? t.motion_permille : kMotionPermille;
-  const double motion = sc.motion_level >= 2 ? 1.0 : (sc.motion_level == 1 ? std::max(pm, 800) : pm) / 1000.0;
+  const double motion = motion_permille_at(t, sc.motion_level) / 1000.0;
   const double L = sc.k.r2b / (1.0 - sc.Minf) * motion;
   const int W = t.window;
   const float* P = sc.pos->data();
@@ -990,9 +1002,15 @@ int check_flags(acum_ctx* ctx, Stage& st, int prec, bool* retry) {
   HIPCHK(ctx, hipMemcpy(f.data(), st.plan[prec].d_flags, nc * sizeof(uint32_t), hipMemcpyDeviceToHost));
   *retry = false;
   uint32_t all = 0;
+  const Plan& p = st.plan[prec];
   for (int c = 0; c < nc; c++) {
-    if ((f[c] & kFlagWindow) && st.cases[c].motion_level < 2) {
-      st.cases[c].motion_level++;
+    StagedCase& sc = st.cases[c];
+    if ((f[c] & kFlagWindow) && sc.motion_level < kMotionLevels - 1) {
+      const acum_tune& t = p.tunes[c];
+      const int cur = motion_permille_at(t, sc.motion_level);
+      do
+        sc.motion_level++;
+      while (sc.motion_level < kMotionLevels - 1 && motion_permille_at(t, sc.motion_level) <= cur);
       *retry = true;
       f[c] &= ~kFlagWindow;
     }
@@ -1099,12 +1117,12 @@ int run_stage_once(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_gra
 // Tiles are planned with a reduced bound on how far a source moves in observer time per step
 // (acum_tune.motion_permille), which lengthens the passes; the kernel checks every footprint
 // against its window, so a case whose bound was too tight is planned again with a wider bound
-// (at least 0.8, then the full one) and the pass is run again.  The plan of a case depends only on the case, so results agree
-// between a case run alone and in a batch.
+// (the next of 0.6, 0.8 and the full bound that widens it) and the pass is run again.  The plan
+// of a case depends only on the case, so results agree between a case run alone and in a batch.
 int run_stage(acum_ctx* ctx, Stage& st, int prec, int repeats, bool use_graph) {
   bool retry = false;
   int rc = run_stage_once(ctx, st, prec, repeats, use_graph, &retry);
-  for (int attempt = 0; rc == ACUM_OK && retry && attempt < 2; attempt++) {
+  for (int attempt = 0; rc == ACUM_OK && retry && attempt < kMotionLevels - 1; attempt++) {
     for (auto& p : st.plan) free_plan(p);  // both precisions' plans hold the cases' tiles
     rc = run_stage_once(ctx, st, prec, repeats, use_graph, &retry);
   }
@@ -1136,6 +1154,8 @@ acum_tune default_tune(int prec) {
 // passes, and the fastest of them wins.  A plan's first pass is timed like the others (its
 // kernels are loaded and its buffers allocated before the timed events).  Windows above 24 bins
 // are left out: their LDS halves the kernel's occupancy, which has never paid.
+// The winner then runs once on the whole case, which keeps its motion bound only if the bound
+// holds for every observer.
 // The view's size sets how well its timings rank shapes for the whole case: on the four-blade
 // rotor and four of the seeded uneven cases, shapes tuned on 512 observers ran 1.5-6.7% faster
 // than the default shape on every case, while 128 and 256 observers picked shapes up to 2.7% and
@@ -1197,6 +1217,22 @@ int tune_case(acum_ctx* ctx, const StagedCase& sc, int prec, TuneEntry* out, int
   for (auto& p : view.plan) free_plan(p);
   if (rc) return rc;
   if (cand.empty() || !(cand[0].ms < 1e300)) return ctx_fail(ctx, ACUM_ERR_PLAN, "no candidate tile shape ran");
+  // The winner's motion bound held for the view's observers; one pass of the whole case checks
+  // it for all of them, and a bound that overflowed there is replaced by the one that held, so
+  // later plans in the bucket do not pay the rerun.
+  {
+    Stage whole;
+    whole.cases.push_back(sc);
+    StagedCase& w = whole.cases[0];
+    w.forced = true;
+    w.force = cand[0].t;
+    w.motion_level = 0;
+    rc = run_stage(ctx, whole, prec, 1, false);
+    for (auto& p : whole.plan) free_plan(p);
+    *passes += 1 + w.motion_level;
+    if (rc) return rc;
+    if (w.motion_level > 0) cand[0].t.motion_permille = motion_permille_at(cand[0].t, w.motion_level);
+  }
   out->tune = cand[0].t;
   out->tune.waves = 1;
   out->ms = cand[0].ms;

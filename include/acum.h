@@ -130,8 +130,9 @@ typedef struct acum_tune {
                        /* the LDS window, not the workgroup size, sets the occupancy)         */
   int32_t motion_permille; /* planning bound on observer-time change per metre of source    */
                        /* motion, per mille of the worst case (0 = default 600); a case whose */
-                       /* window overflows under it is planned again with at least 800, then  */
-                       /* the full bound, and run again within the same call                  */
+                       /* window overflows under it is planned again with the next of 600,    */
+                       /* 800 and the full bound that widens it, and run again within the      */
+                       /* same call                                                            */
 } acum_tune;
 
 /* --- context ----------------------------------------------------------------------------- */
@@ -173,8 +174,11 @@ int acum_set_tuning(acum_ctx* ctx, int32_t precision, const acum_tune* t);
  * quarter per round with twice the passes.  The chosen shape is cached by bucket, and every
  * later plan of a case in that bucket (alone or in any batch) uses it, so batched and single
  * runs of a case stay bit-identical.
+ * The winner then runs once on the whole case and keeps its source-motion bound only if that
+ * holds for every observer (else the bound that held replaces it).
  * best / best_ms: the shape and time per pass (on the observer subset) of case 0's bucket.
- * acum_get_timing then reports tune_cases, tune_hits and tune_passes of the call.             */
+ * acum_get_timing then reports tune_cases, tune_hits and tune_passes of the call.  The tuning
+ * passes overwrite the staged cases' histories: run the stage again before fetching them.     */
 int acum_autotune(acum_ctx* ctx, int32_t handle, int32_t precision, acum_tune* best, double* best_ms);
 /* The shape case icase of a staged batch plans with for a precision (from the cache or the
  * context's shape).                                                                          */

@@ -186,14 +186,14 @@ def test_c1_full_size(ctx):
 def test_window_overflow_widens_the_motion_bound(ctx, prec):
     """Planned with a far too small bound on observer-time change per step, the passes overflow
     their windows; the kernel's window check catches it and the call plans the case again with a
-    wider bound (0.8 of the worst case, then the full one) and runs it again.  The results are
-    bit-identical to planning with that bound from the start (a plan that kept the small bound
-    would group the sums differently) and agree with the oracle."""
+    wider bound (0.6 of the worst case, then 0.8, then the full one) and runs it again.  The
+    results are bit-identical to planning with that bound from the start (a plan that kept the
+    small bound would group the sums differently) and agree with the oracle."""
     keep = ctx.get_tuning(prec)
     try:
         run, c = small_case(2, prec)
         refs = []
-        for pm in (800, 1000):
+        for pm in (600, 800, 1000):
             ctx.set_tuning(prec, motion_permille=pm)
             refs.append(ctx.tonal(c))
         ctx.set_tuning(prec, motion_permille=100)
@@ -216,7 +216,7 @@ def test_autotune_picks_a_candidate_and_leaves_the_stage_consistent(ctx):
         h = ctx.stage([c])
         best, ms = ctx.autotune(h, acum.MIXED)
         assert best.window in (16, 20, 24) and best.max_spread in (2, 3, 4) and ms > 0
-        assert best.max_tile in (64, 256) and best.motion_permille in (500, 600, 800)
+        assert best.max_tile in (64, 256) and best.motion_permille in (500, 600, 800, 1000)
         ctx.run(h, acum.MIXED)
         tuned = ctx.fetch(h, 0, c.nTime, c.nObs)
         ctx.release(h)
