@@ -91,6 +91,7 @@ def _load():
         "acum_ctx_destroy": (None, [P]),
         "acum_last_error": (C.c_char_p, [P]),
         "acum_tonal": (C.c_int, [P, C.POINTER(TonalCase), _f32p, _f32p, _f32p]),
+        "acum_tonal_plain": (C.c_int, [P, C.POINTER(TonalCase), _f32p, _f32p, _f32p]),
         "acum_tonal_batch": (C.c_int, [P, C.POINTER(TonalCase), C.c_int32, C.POINTER(_f32p),
                                        C.POINTER(_f32p), C.POINTER(_f32p)]),
         "acum_tonal_stage": (C.c_int, [P, C.POINTER(TonalCase), C.c_int32, _i32p]),
@@ -141,6 +142,7 @@ lib = _load()
 
 # Names every C-ABI entry point declared in include/acum.h.
 EXPORTS = ("acum_device_count", "acum_ctx_create", "acum_ctx_destroy", "acum_last_error", "acum_tonal",
+           "acum_tonal_plain",
            "acum_tonal_batch", "acum_tonal_stage", "acum_tonal_run", "acum_tonal_fetch",
            "acum_tonal_release", "acum_tonal_bins", "acum_get_timing", "acum_get_tuning",
            "acum_set_tuning", "acum_autotune", "acum_bpm", "acum_host_error", "acum_run_load",
@@ -309,6 +311,14 @@ class Context:
         n = case.nTime * case.nObs
         out = [np.empty(n, np.float32) for _ in range(3)]
         self._check(lib.acum_tonal(self._h, C.byref(case), *[fptr(a) for a in out]))
+        return [a.reshape(case.nTime, case.nObs) for a in out]
+
+    def tonal_plain(self, case):
+        """The plain FP64 path (the reference's kernel structure in FP64, the measurement
+        baseline); the case's precision field is not used."""
+        n = case.nTime * case.nObs
+        out = [np.empty(n, np.float32) for _ in range(3)]
+        self._check(lib.acum_tonal_plain(self._h, C.byref(case), *[fptr(a) for a in out]))
         return [a.reshape(case.nTime, case.nObs) for a in out]
 
     def tonal_batch(self, cases):

@@ -28,13 +28,18 @@ hipError_t launch_tonal_reduce(bool mixed, hipStream_t s, const TonalCaseDev* ca
 hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, const TimeConst& k, void* pos);
 hipError_t launch_tonal_attr(bool mixed, hipStream_t s, const float* X, const float* N,
                              const float* V, const float* PR, int64_t n, double a0, double rhoRef,
-                             void* attr, uint32_t* flags);
+                             double KB, void* attr, uint32_t* flags);
 hipError_t launch_fill_zero(hipStream_t s, void* p, int64_t bytes);
 hipError_t launch_reset(hipStream_t s, int* counter, uint32_t* flags);
 hipError_t launch_tonal_bins(hipStream_t s, const void* pos, int64_t nE, const TimeConst& k,
                              double ox, double oy, double oz, int32_t* ipts,
                              int32_t* npi, int32_t* nd, int32_t* nu);
 hipError_t spectra_scratch_init(int device);
+hipError_t launch_tonal_plain(hipStream_t s, const float* X, const float* N, const float* V, const float* PR,
+                              const float* obs, int64_t nE, int nT, int nObs, double dTau, double dPsi,
+                              double a0, double rhoRef, double oM, const double* vInf, double* acc,
+                              uint32_t* flags);
+hipError_t launch_plain_to_float(hipStream_t s, const double* acc, float* const* out, int64_t n, uint32_t* flags);
 }  // namespace acum
 
 using namespace acum;
@@ -609,7 +614,7 @@ int ensure_records(acum_ctx* ctx, StagedCase& sc, int prec) {
   const size_t bytes = (size_t)n * (mixed ? sizeof(AttrT<float>) : sizeof(AttrT<double>));
   HIPCHK(ctx, hipMalloc(&sc.d_attr[prec], bytes));
   HIPCHK(ctx, launch_tonal_attr(mixed, ctx->stream, sc.d_aos[0], sc.d_aos[1], sc.d_aos[2], sc.d_aos[3],
-                                n, sc.k.a0, sc.rhoRef, sc.d_attr[prec], ctx->d_flags));
+                                n, sc.k.a0, sc.rhoRef, sc.k.KB, sc.d_attr[prec], ctx->d_flags));
   {
     uint32_t f = 0;
     HIPCHK(ctx, hipMemcpyAsync(&f, ctx->d_flags, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
@@ -1398,6 +1403,53 @@ int acum_tonal(acum_ctx* ctx, const acum_tonal_case* c, float* pT, float* pL, fl
   if (!rc) rc = acum_tonal_fetch(ctx, h, 0, pT, pL, pA);
   acum_tonal_release(ctx, h);
   return rc;
+}
+
+int acum_tonal_plain(acum_ctx* ctx, const acum_tonal_case* c, float* pT, float* pL, float* pA) {
+  if (!ctx || !c || !pT || !pL || !pA) return ctx_fail(ctx, ACUM_ERR_ARG, "bad argument");
+  Stage st;  // one staged case (the reference AoS arrays on the device), freed on every return
+  struct Release {
+    Stage& s;
+    double* acc = nullptr;
+    ~Release() {
+      (void)hipFree(acc);
+      free_stage(s);
+    }
+  } rel{st};
+  st.cases.resize(1);
+  StagedCase& sc = st.cases[0];
+  int rc = stage_case(ctx, c, sc);
+  if (rc) return rc;
+  const int64_t n = sc.nT * sc.nObs;
+  HIPCHK(ctx, hipMalloc(&rel.acc, (size_t)n * 3 * sizeof(double)));
+  HIPCHK(ctx, launch_reset(ctx->stream, ctx->d_counter, ctx->d_flags));
+  HIPCHK(ctx, launch_fill_zero(ctx->stream, rel.acc, n * 3 * (int64_t)sizeof(double)));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+  HIPCHK(ctx, launch_tonal_plain(ctx->stream, sc.d_aos[0], sc.d_aos[1], sc.d_aos[2], sc.d_aos[3], sc.d_obs,
+                                 sc.nE, (int)sc.nT, (int)sc.nObs, c->dTau, c->dPsi, c->a0, c->rhoRef, c->oM,
+                                 c->vInf, rel.acc, ctx->d_flags));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  HIPCHK(ctx, launch_plain_to_float(ctx->stream, rel.acc, sc.d_out, n, ctx->d_flags));
+  HIPCHK(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+  uint32_t f = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&f, ctx->d_flags, sizeof(f), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  float ev = 0, cv = 0;
+  HIPCHK(ctx, hipEventElapsedTime(&ev, ctx->ev[1], ctx->ev[2]));
+  HIPCHK(ctx, hipEventElapsedTime(&cv, ctx->ev[2], ctx->ev[3]));
+  acum_timing& t = ctx->timing;
+  t = acum_timing();
+  t.eval_ms = ev;
+  t.reduce_ms = cv;
+  t.total_ms = ev + cv;
+  t.evals = sc.nE * sc.nT * sc.nObs;
+  t.launches = 5;
+  t.eval_launches = 1;
+  if (f & kFlagNumeric)
+    return ctx_fail(ctx, ACUM_ERR_NUMERIC, "plain path: r (1 - Mr) <= 0, observer time not increasing or a non-finite result");
+  float* out[3] = {pT, pL, pA};
+  for (int i = 0; i < 3; i++) HIPCHK(ctx, ctx_download(ctx, out[i], sc.d_out[i], (size_t)n * sizeof(float)));
+  return ACUM_OK;
 }
 
 int acum_tonal_batch(acum_ctx* ctx, const acum_tonal_case* cases, int32_t nCases, float* const* pT,

@@ -14,7 +14,9 @@ namespace acum {
 //   l   = cL P n,  ld = cL (dP/dt n + P dn/dt)             (L, dL/dt of core.cu:130-131)
 //   aM1 = a0 (M^2 - 1) = |V|^2 / a0 - a0,  aLM = cL a0 L_M = cL P (V.n)   (core.cu:94,134)
 //   cT  = rhoRef (V.n) dS a0 D1,  cL = dS D1,  D1 = 1/(4 pi a0)   (core.cu:53,123-124,138-141)
-// (the loading factor cL is folded into l, ld and aLM, which only enter the loading term)
+// (the loading factor cL is folded into l, ld and aLM, which only enter the loading term), each
+// stored scaled by s = KB / a0 (v, vd, aM1, cT) or s^2 (l, ld, aLM), so that the eval kernel's
+// integrand takes the delay in bins T where it would take r = (a0 / KB) T
 struct PosRec {
   double mx, my, mz;  // (-2 C2) (p - O)
   double q;           // C2 |p - O|^2

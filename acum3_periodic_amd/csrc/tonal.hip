@@ -142,15 +142,16 @@ struct Prec<true> {
 //   pThick = cT g^3 (r (dVd - a0 M^2) + a0 dV) = cT g^2 (r t - a0)
 //   pLoad  = g^2 (dL g (dVd - a0 M^2 + a0) + dLd - cL a0 L_M) = g^2 (dL t + dLd - aLM)
 // with w = dVd - a0 (M^2 - 1) and t = g w (using 1 + g dV = r g and g (r - dV) = 1), one
-// reciprocal per eval.  d is obs - src and `del` the delay T in bins: r = aB T and r_vec = d - u T (core.cu:86;
-// aB = a0 / KB, u = vinf / KB; u = 0 in still air).  Returns r (1 - Mr): positive for a valid
-// eval, zero for a source on the observer (r = 0 gives r_vec = 0 and dV = 0) and negative for
-// Mr > 1.
-// r and r_vec of an eval
+// reciprocal per eval.  d is obs - src and `del` the delay T in bins: r = aB T and r_vec = d - u T
+// (core.cu:86; aB = a0 / KB, u = vinf / KB; u = 0 in still air).  The records are stored scaled by
+// s = 1 / aB (v, vd, aM1 and cT by s; l, ld and aLM by s^2), which makes r = T, a0 = KB and
+// g = 1 / (T - dV) = aB / (r (1 - Mr)) in the same formulas with the same pThick and pLoad, and
+// saves the product aB T.  Returns (T - dV) = r (1 - Mr) / aB: positive for a valid eval, zero
+// for a source on the observer (T = 0 gives r_vec = 0 and dV = 0) and negative for Mr > 1.
+// r_vec of an eval
 template <typename R, bool MOVING>
-__device__ __forceinline__ void radiation_vector(R dx, R dy, R dz, R del, R aB, R vx, R vy, R vz,
-                                                 R* r, R* rx, R* ry, R* rz) {
-  *r = aB * del;
+__device__ __forceinline__ void radiation_vector(R dx, R dy, R dz, R del, R vx, R vy, R vz, R* rx,
+                                                 R* ry, R* rz) {
   if (MOVING) {
     *rx = fmaT(-vx, del, dx);
     *ry = fmaT(-vy, del, dy);
@@ -162,29 +163,29 @@ __device__ __forceinline__ void radiation_vector(R dx, R dy, R dz, R del, R aB, 
   }
 }
 
-// the integrand from r and the four record products
+// the integrand from the delay T and the four record products (scaled records, a0 -> KB)
 template <typename R>
-__device__ __forceinline__ R integrand_tail(R aM1, R aLM, R cT, R r, R dV, R dVd, R dL, R dLd, R a0,
+__device__ __forceinline__ R integrand_tail(R aM1, R aLM, R cT, R del, R dV, R dVd, R dL, R dLd, R KB,
                                             R* pth, R* plo) {
-  const R rm = r - dV;                // r (1 - Mr)
+  const R rm = del - dV;              // r (1 - Mr) / aB
   const R g = rcpT(rm);
   const R g2 = g * g;
   const R t = g * (dVd - aM1);        // g w, shared by both terms
-  *pth = (cT * g2) * fmaT(r, t, -a0);
+  *pth = (cT * g2) * fmaT(del, t, -KB);
   *plo = g2 * fmaT(dL, t, dLd - aLM);
   return rm;
 }
 
 template <typename R, bool MOVING>
-__device__ __forceinline__ R integrand(const AttrT<R>& q, R dx, R dy, R dz, R del, R a0, R aB,
-                                       R vx, R vy, R vz, R* pth, R* plo) {
-  R r, rx, ry, rz;
-  radiation_vector<R, MOVING>(dx, dy, dz, del, aB, vx, vy, vz, &r, &rx, &ry, &rz);
+__device__ __forceinline__ R integrand(const AttrT<R>& q, R dx, R dy, R dz, R del, R KB, R vx, R vy,
+                                       R vz, R* pth, R* plo) {
+  R rx, ry, rz;
+  radiation_vector<R, MOVING>(dx, dy, dz, del, vx, vy, vz, &rx, &ry, &rz);
   const R dV = fmaT(q.v[2], rz, fmaT(q.v[1], ry, q.v[0] * rx));
   const R dVd = fmaT(q.vd[2], rz, fmaT(q.vd[1], ry, q.vd[0] * rx));
   const R dL = fmaT(q.l[2], rz, fmaT(q.l[1], ry, q.l[0] * rx));
   const R dLd = fmaT(q.ld[2], rz, fmaT(q.ld[1], ry, q.ld[0] * rx));
-  return integrand_tail<R>(q.aM1, q.aLM, q.cT, r, dV, dVd, dL, dLd, a0, pth, plo);
+  return integrand_tail<R>(q.aM1, q.aLM, q.cT, del, dV, dVd, dL, dLd, KB, pth, plo);
 }
 
 // Mixed path: the four record products of 64 observers by the matrix cores.  With
@@ -215,15 +216,15 @@ __device__ __forceinline__ MixRec load_mixrec(const AttrT<float>* rec, int lane)
 
 template <bool MOVING>
 __device__ __forceinline__ float integrand_mfma(const MixRec& q, float dx, float dy, float dz, float del,
-                                                float a0, float aB, float vx, float vy, float vz,
-                                                float* pth, float* plo) {
-  float r, rx, ry, rz;
-  radiation_vector<float, MOVING>(dx, dy, dz, del, aB, vx, vy, vz, &r, &rx, &ry, &rz);
+                                                float KB, float vx, float vy, float vz, float* pth,
+                                                float* plo) {
+  float rx, ry, rz;
+  radiation_vector<float, MOVING>(dx, dy, dz, del, vx, vy, vz, &rx, &ry, &rz);
   v4f d = {0.0f, 0.0f, 0.0f, 0.0f};
   d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.x, rx, d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.y, ry, d, 0, 0, 0);
   d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.z, rz, d, 0, 0, 0);
-  return integrand_tail<float>(q.t.aM1, q.t.aLM, q.t.cT, r, d[0], d[1], d[2], d[3], a0, pth, plo);
+  return integrand_tail<float>(q.t.aM1, q.t.aLM, q.t.cT, del, d[0], d[1], d[2], d[3], KB, pth, plo);
 }
 
 // The high word of a finite value read as a signed integer is positive exactly when the value is
@@ -237,6 +238,12 @@ __device__ __forceinline__ int sign_word(double x) { return (int)(__double_as_lo
 __device__ __forceinline__ int min3i(int a, int b, int c) {
   int r;
   __asm__("v_min3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+// x clamped into [lo, hi] (lo <= hi) in one instruction
+__device__ __forceinline__ int med3i(int x, int lo, int hi) {
+  int r;
+  __asm__("v_med3_i32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
   return r;
 }
 
@@ -287,6 +294,8 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
   // win[slot][lane] = (thickness, loading) partial of one observer-time bin
   __shared__ R2 win[W * 64];
   const int lane = threadIdx.x;
+  // the lane's byte addresses of window slots 0 and W - 4 (a footprint's first slot lies between)
+  const int wlo = lane * (int)sizeof(R2), whi = ((W - 4) * 64 + lane) * (int)sizeof(R2);
   for (int s = 0; s < W; s++) win[s * 64 + lane] = R2{(R)0, (R)0};
   // Errors: `valid` is the minimum sign word of r (1 - Mr) and of every step tau_next - tau, so
   // it stays positive while every eval is valid and observer time increases.  Increasing time
@@ -319,7 +328,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
     R* slab = (R*)Cp->slab + ((int64_t)item.group * nObs + oc) * (int64_t)nT * 2;
     const PosPtr pos = (PosPtr)(uintptr_t)Cp->pos;
     const AttrPtr attr = (AttrPtr)(uintptr_t)Cp->attr;
-    const R a0 = (R)k.a0, aB = (R)k.aB, ux = (R)k.ux, uy = (R)k.uy, uz = (R)k.uz;
+    const R KB = (R)k.KB, ux = (R)k.ux, uy = (R)k.uy, uz = (R)k.uz;
     const double dnT = (double)nT;
     // obs - src of the radiation vector: in FP32 from the parsed floats (mixed; the same bits as
     // the FP64 difference rounded to float), in FP64 as o + h2 m (FP64 path)
@@ -356,6 +365,12 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
           if (k0 == 0) tb = tb - dnT;
           base = (int)floor(tb) - tl.spread - 1;
         }
+        // byte address in the window of a footprint's first slot ic - 1 - base, before clamping:
+        // ic (64 sizeof(R2)) + wofs
+        int wofs = (lane - 64 * (base + 1)) * (int)sizeof(R2);
+        // kept opaque, so that each step forms the address in one shift-add (the compiler would
+        // otherwise rebuild it from its parts with an extra instruction per step)
+        if constexpr (MIXED) __asm__("" : "+v"(wofs));
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
           const PosPtr pe = pos + e * nT;
           const AttrPtr ae = attr + e * nT;
@@ -372,6 +387,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             double t;  // observer time tau in bins
             int ic;    // floor(tau)
             R id;      // 1 / (tau - tau of the step before)
+            R om;      // 1 - id
           };
           Step s0, s1;
           R dxc, dyc, dzc, delc;  // obs - src and the delay of the step being evaluated
@@ -383,10 +399,11 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             diff(pc_e, &dxc, &dyc, &dzc);
             delc = (R)dl;
             const R dp = (R)(tc - tp);
-            s1 = Step{tp, (int)floor(tp), (R)0};
+            s1 = Step{tp, (int)floor(tp), (R)0, (R)1};
             // tau >= 0 from here on (only the step before step 0 carries the -nT wrap), so
             // truncation is floor
-            s0 = Step{tc, (int)tc, rcpT(dp)};
+            const R idp = rcpT(dp);
+            s0 = Step{tc, (int)tc, idp, (R)1 - idp};
             valid = min(valid, sign_word(dp));
           }
           winmask |= __ballot(s0.ic - 1 - base < 0);  // the pass's first footprint
@@ -414,9 +431,12 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             pnn = load_words<PosRec>(pe + knn);
             an = load_rec(kn);
             // the footprint is known before the solve: start its window reads now
+            // (the footprint's first slot ic - 1 - base clamped into [0, W - 4], as a byte
+            // address: the clamp bounds are the lane's addresses of slots 0 and W - 4)
+            // (the FP64 path, short of registers, clamps the slot index instead)
             const int ic = cur.ic;
-            const int sb = min(max(ic - 1 - base, 0), W - 4);
-            R2* wp = &win[sb * 64 + lane];
+            R2* wp = MIXED ? (R2*)((char*)win + med3i(ic * (64 * (int)sizeof(R2)) + wofs, wlo, whi))
+                           : &win[min(max(ic - 1 - base, 0), W - 4) * 64 + lane];
             R2 v0 = wp[0], v1 = wp[64], v2 = wp[128], v3 = wp[192];
             // keep the record loads and window reads at the top of the step: their latency then
             // overlaps the solve and the integrand instead of the window update
@@ -429,19 +449,22 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             R pth, plo;
             R rm;
             if constexpr (MIXED)
-              rm = integrand_mfma<MOVING>(ac, dxc, dyc, dzc, delc, a0, aB, ux, uy, uz, &pth, &plo);
+              rm = integrand_mfma<MOVING>(ac, dxc, dyc, dzc, delc, KB, ux, uy, uz, &pth, &plo);
             else
-              rm = integrand<R, MOVING>(ac, dxc, dyc, dzc, delc, a0, aB, ux, uy, uz, &pth, &plo);
+              rm = integrand<R, MOVING>(ac, dxc, dyc, dzc, delc, KB, ux, uy, uz, &pth, &plo);
             // tent weights of the footprint
             const R fr = fractT(cur.t, R());  // tau - ic, exact
             const R dn = (R)(t_next - cur.t);
             valid = min3i(valid, sign_word(rm), sign_word(dn));
             const R idp = cur.id, idn = rcpT(dn);
+            const R omn = (R)1 - idn;
             const R frm = fr - (R)1;
-            const R w0 = clamp01(fmaT(-fr, idp, (R)1 - idp));  // bin ic-1: 1 - (fr + 1) / dp
+            // (1 - idp carried from the step before in the mixed path; the FP64 path, short of
+            // registers, forms it again)
+            const R w0 = clamp01(fmaT(-fr, idp, MIXED ? cur.om : (R)1 - idp));  // bin ic-1: 1 - (fr + 1) / dp
             const R w1 = clamp01(fmaT(-fr, idp, (R)1));        // bin ic:   1 - fr / dp
             const R w2 = clamp01(fmaT(frm, idn, (R)1));        // bin ic+1: 1 - (1 - fr) / dn
-            const R w3 = clamp01(fmaT(frm, idn, (R)1 - idn));  // bin ic+2: 1 - (2 - fr) / dn
+            const R w3 = clamp01(fmaT(frm, idn, omn));         // bin ic+2: 1 - (2 - fr) / dn
             v0.x = fmaT(pth, w0, v0.x);
             v0.y = fmaT(plo, w0, v0.y);
             v1.x = fmaT(pth, w1, v1.x);
@@ -489,6 +512,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             prv.t = t_next;
             prv.ic = (int)t_next;
             prv.id = idn;
+            prv.om = omn;
             diff(pn, &dxc, &dyc, &dzc);
             delc = (R)dln;
           };
@@ -694,13 +718,16 @@ __global__ void tonal_pos_kernel(const float* __restrict__ X, int64_t n, TimeCon
   }
 }
 
+// The records are stored scaled for the eval kernel's integrand (which takes the delay in bins
+// for r): v, vd, aM1 and cT times s = KB / a0 = 1 / aB, l, ld and aLM times s^2.
 template <typename R>
 __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __restrict__ N,
                                   const float* __restrict__ V, const float* __restrict__ PR,
-                                  int64_t n, double a0, double rhoRef, AttrT<R>* __restrict__ attr,
+                                  int64_t n, double a0, double rhoRef, double KB, AttrT<R>* __restrict__ attr,
                                   uint32_t* __restrict__ flags) {
   bool bad = false;
-  const double inv_a0 = 1.0 / a0;
+  const double sc = KB / a0, sc2 = sc * sc;
+  const double inv_a0 = sc / a0;
   const double D1 = 1.0 / (4.0 * M_PI * a0);
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n;
        q += (int64_t)gridDim.x * blockDim.x) {
@@ -713,7 +740,7 @@ __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __re
     for (int i = 0; i < 4; i++) bad |= !isfinite(x[i]);
     AttrT<R> o;
     double Vn = 0.0, VV = 0.0;
-    const double cL = ds * D1;
+    const double cL = ds * D1 * sc2;
     for (int i = 0; i < 3; i++) {
       const double Vi = v[i], Vdi = v[3 + i], ni = nn[i], ndi = nn[3 + i];
       o.v[i] = (R)(Vi * inv_a0);
@@ -723,9 +750,9 @@ __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __re
       Vn += Vi * ni;
       VV += Vi * Vi;
     }
-    o.aM1 = (R)(VV * inv_a0 - a0);
+    o.aM1 = (R)((VV / a0 - a0) * sc);
     o.aLM = (R)(cL * (P * Vn));
-    o.cT = (R)(rhoRef * Vn * ds * a0 * D1);
+    o.cT = (R)(rhoRef * Vn * ds * a0 * D1 * sc);
     o.unused = (R)0;
     attr[q] = o;
   }
@@ -856,14 +883,14 @@ hipError_t launch_tonal_pos(hipStream_t s, const float* X, int64_t n, const Time
 
 hipError_t launch_tonal_attr(bool mixed, hipStream_t s, const float* X, const float* N,
                              const float* V, const float* PR, int64_t n, double a0, double rhoRef,
-                             void* attr, uint32_t* flags) {
+                             double KB, void* attr, uint32_t* flags) {
   const int grid = (int)std::min<int64_t>((n + 255) / 256, 65536);
   if (mixed)
     hipLaunchKernelGGL(tonal_attr_kernel<float>, dim3(grid), dim3(256), 0, s, X, N, V, PR, n, a0,
-                       rhoRef, (AttrT<float>*)attr, flags);
+                       rhoRef, KB, (AttrT<float>*)attr, flags);
   else
     hipLaunchKernelGGL(tonal_attr_kernel<double>, dim3(grid), dim3(256), 0, s, X, N, V, PR, n, a0,
-                       rhoRef, (AttrT<double>*)attr, flags);
+                       rhoRef, KB, (AttrT<double>*)attr, flags);
   return hipGetLastError();
 }
 

@@ -63,7 +63,7 @@ TONAL_PROFILE = "rotor_4x20000_elements_1024_steps_1024_observers"
 BPM_PROFILE = "4x60_sections_360_steps_1024_observers_34_bands"
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", f"tonal_eval_hbm_traffic_{TONAL_PROFILE}.md")
 # the headline leg alone (the profiled runs leave the other legs out)
-TONAL_ONLY = "--no-bpm --no-batched --no-stress"
+TONAL_ONLY = "--no-bpm --no-batched --no-stress --no-plain"
 EVAL_KERNEL = "tonal_eval_kernel"
 CALIB_BYTES = 1 << 30
 
@@ -402,14 +402,15 @@ def leg_bpm(acum, O, ctx, steps, host):
     return out
 
 
-def leg_batched(acum, O, P, ctx, host, ncases, per_launch, oracle_obs, steps=2, fp64=True):
+def leg_batched(acum, O, P, ctx, host, ncases, per_launch, oracle_obs, steps=2, fp64=True, plain=True):
     """The batched workload's first `ncases` seeded rotor cases (of 64: 2-6 blades, 2,000-40,000
     elements per blade, 256-2048 steps, 16-2048 observers on spheres of 5R-50R, oM 150-400,
     Minf 0-0.2), `per_launch` cases per persistent launch: tile shapes per case from the
     autotuner, the timed passes replayed as a captured HIP graph, the FP64 path on the same
     launches, every case checked bit for bit against it run alone and against the FP64 oracle on
-    its first observers."""
+    its first observers; the plain FP64 path timed on a sample of the first case."""
     seed = 31004
+    plain_rate, plain_sample = None, None
     tot_evals = 0
     mixed_ms = fp64_ms = eval_ms = 0.0
     host_s = tune_s = 0.0
@@ -461,6 +462,16 @@ def leg_batched(acum, O, P, ctx, host, ncases, per_launch, oracle_obs, steps=2, 
                 _, cpu_s, _ = timed_runs(lambda: O.tonal_f64(cb, host["threads"]), 3)
                 cpu_evals = sum(c.E[s] for s in range(c.nSurf)) * c.nTime * cb.nObs
                 cpu_obs = cb.nObs
+                if plain:
+                    # the plain FP64 path on about 2e10 evals of the first case (its first
+                    # observers; a pass of the whole case would take tens of seconds)
+                    ne = sum(c.E[s] for s in range(c.nSurf)) * c.nTime
+                    nop = int(min(c.nObs, max(64, round(2e10 / ne))))
+                    ctx.tonal_plain(P.obs_subset(c, np.arange(nop), acum.FP64))
+                    tp = ctx.timing()
+                    plain_rate = tp.evals / (tp.eval_ms * 1e-3) / 1e9
+                    plain_sample = (f"first {nop} observers of case 0 ({tp.evals:.3g} evals, one pass, "
+                                    f"{tp.eval_ms / 1e3:.1f} s)")
         note(f"batched cases {idx[0]}-{idx[-1]}: {ev:.3g} evals, {tm.total_ms:.1f} ms per mixed pass")
         del runs, cases, batch_out
     value = tot_evals / (mixed_ms * 1e-3) / 1e9
@@ -489,6 +500,10 @@ def leg_batched(acum, O, P, ctx, host, ncases, per_launch, oracle_obs, steps=2, 
     if fp64:
         out["fp64_value"] = round(tot_evals / (fp64_ms * 1e-3) / 1e9, 3)
         out["mixed_over_fp64"] = round(value / out["fp64_value"], 3)
+    if plain:
+        out["plain_fp64_value"] = round(plain_rate, 3)
+        out["plain_fp64_sample"] = plain_sample
+        out["mixed_over_plain_fp64"] = round(value / plain_rate, 2)
     return out
 
 
@@ -550,6 +565,8 @@ def parity_failures(out):
         bad.append("rotor mixed 1/3-octave band")
     if acc.get("fp64_vs_oracle_rel_l2_pA", 0) > 1e-6:
         bad.append("rotor FP64 rel-L2")
+    if acc.get("plain_fp64_vs_oracle_rel_l2_pA", 0) > 1e-6:
+        bad.append("rotor plain FP64 rel-L2")
     for leg in ("batched", "stress"):
         a = (out.get(leg) or {}).get("accuracy") or {}
         if a.get("max_doaspl_db", 0) > 0.01 or a.get("max_dband_db", 0) > 0.01:
@@ -571,6 +588,7 @@ def main():
     ap.add_argument("--no-bpm", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-stress", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="leave out the plain FP64 path")
     ap.add_argument("--graph", action="store_true", help="replay the pass as a captured HIP graph")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain device copy first (for scaling HBM counters)")
@@ -606,7 +624,8 @@ def main():
     if args.workload in ("batched", "stress", "bpm"):
         ctx = acum.Context(0)
         if args.workload == "batched":
-            out = leg_batched(acum, O, P, ctx, host, args.cases, args.wave, 4, max(1, args.steps), not args.no_fp64)
+            out = leg_batched(acum, O, P, ctx, host, args.cases, args.wave, 4, max(1, args.steps), not args.no_fp64,
+                              not args.no_plain)
         elif args.workload == "stress":
             out = leg_stress(acum, O, P, ctx, host, args.steps, args.oracle_obs)
         else:
@@ -691,6 +710,20 @@ def main():
                     eval_ms=round(e64, 3), reduce_ms=round(r64, 4), partials=t64.n_partials,
                     roofline_frac=round(FLOP_PER_EVAL * evals / (e64 * 1e-3) / PEAK_FP64, 4))
     ctx.release(h)
+    plain_fp64 = plain_out = None
+    if not args.no_plain:
+        note("plain FP64 path (the reference's kernel structure in FP64), one pass")
+        t0 = time.time()
+        plain_out = ctx.tonal_plain(run.tonal_case(acum.FP64))
+        call_s = time.time() - t0
+        tpl = ctx.timing()
+        plain_fp64 = dict(kernel="tonal_plain_kernel", value=round(evals / (tpl.eval_ms * 1e-3) / 1e9, 3),
+                          eval_ms=round(tpl.eval_ms, 1), call_s=round(call_s, 2),
+                          roofline_frac=round(FLOP_PER_EVAL * evals / (tpl.eval_ms * 1e-3) / PEAK_FP64, 5),
+                          structure="one thread per (step, observer), 8x4x8 blocks striding over elements "
+                                    "(the reference's launch), three retarded-time solves per eval, FP64 "
+                                    "atomic adds into pT/pL/pA",
+                          timed="one pass, kernel time by HIP events (element data resident in HBM)")
 
     # CPU baseline + accuracy: the FP64 oracle on observers spread over the hemisphere (observers
     # are independent, so the rate is exact for the sample), median of three runs; the float
@@ -722,6 +755,11 @@ def main():
         acc["fp64_vs_oracle_rel_l2_pT"] = O.rel_l2(plain[0][:, idx], oT)
         d = np.abs(acum.oaspl(mixed[2]) - acum.oaspl(plain[2]))
         acc["mixed_vs_fp64_all_obs_max_doaspl_db"] = float(d.max())
+    if plain_out is not None:
+        acc["plain_fp64_vs_oracle_rel_l2_pA"] = O.rel_l2(plain_out[2][:, idx], oA)
+        acc["plain_fp64_vs_oracle_rel_l2_pT"] = O.rel_l2(plain_out[0][:, idx], oT)
+        d = np.abs(acum.oaspl(mixed[2]) - acum.oaspl(plain_out[2]))
+        acc["mixed_vs_plain_fp64_all_obs_max_doaspl_db"] = float(d.max())
 
     o_spl, _ = O.tonal_spectra(mixed[2][:, idx], dt, iso)
     spectra = dict(kernels="spectra_stats + spectra_bins + spectra_bands", ms=round(sp_ms, 3), bands=int(iso.size),
@@ -733,7 +771,8 @@ def main():
         bpm = leg_bpm(acum, O, ctx, args.steps, host)
     if not args.no_batched:
         note(f"batched leg: first {args.cases} cases, {args.wave} per launch")
-        batched = leg_batched(acum, O, P, ctx, host, args.cases, args.wave, 4, 2, not args.no_fp64)
+        batched = leg_batched(acum, O, P, ctx, host, args.cases, args.wave, 4, 2, not args.no_fp64,
+                              not args.no_plain)
     if not args.no_stress:
         note("stress leg")
         stress = leg_stress(acum, O, P, ctx, host, 2, 16)
@@ -785,6 +824,9 @@ def main():
                     "partials": partials},
         "fp64_path": fp64,
         "mixed_over_fp64": round(value / fp64["value"], 3) if fp64 else None,
+        # the baseline the path's 10x goal names: the reference's kernel structure in FP64
+        "plain_fp64_path": plain_fp64,
+        "mixed_over_plain_fp64": round(value / plain_fp64["value"], 2) if plain_fp64 else None,
         "accuracy": acc,
         "spectra": spectra,
         "bpm": bpm,

@@ -147,6 +147,14 @@ const char* acum_last_error(const acum_ctx* ctx);
 /* --- tonal ------------------------------------------------------------------------------- */
 /* One case, host in / host out.  Replaces main.cu:156-247 (launch at main.cu:236).           */
 int acum_tonal(acum_ctx* ctx, const acum_tonal_case* c, float* pT, float* pL, float* pA);
+/* The plain FP64 path, kept as the measurement baseline of the restructured paths: the
+ * reference's kernel structure (findPressureTimeHistory, core.cu:18-259, with the launch shape
+ * of main.cu:194-236) in FP64 -- one thread per (step, observer) striding over elements, three
+ * retarded-time solves per eval and FP64 atomic adds into the bins.  Host in / host out like
+ * acum_tonal, same errors; c->precision is not used.  acum_get_timing reports the kernel time
+ * (eval_ms) and the conversion of the FP64 bins to the float outputs (reduce_ms).  Bin sums
+ * depend on the order the atomics are served in, so repeated calls agree to rounding only.    */
+int acum_tonal_plain(acum_ctx* ctx, const acum_tonal_case* c, float* pT, float* pL, float* pA);
 /* Many cases of uneven size in one persistent launch.  Per-case results equal acum_tonal's.  */
 int acum_tonal_batch(acum_ctx* ctx, const acum_tonal_case* cases, int32_t nCases,
                      float* const* pT, float* const* pL, float* const* pA);
