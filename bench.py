@@ -62,9 +62,10 @@ ISO_BANDS = (10, 12.5, 16, 20, 25, 31.5, 40, 50, 63, 80, 100, 125, 160, 200, 250
 TONAL_PROFILE = "rotor_4x20000_elements_1024_steps_1024_observers"
 BPM_PROFILE = "4x60_sections_360_steps_1024_observers_34_bands"
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", f"tonal_eval_hbm_traffic_{TONAL_PROFILE}.md")
-# the headline leg alone (the profiled runs leave the other legs out)
-TONAL_ONLY = "--no-bpm --no-batched --no-stress --no-plain"
+# the rotor leg alone with the plain FP64 path (the profiled runs leave the other legs out)
+TONAL_ONLY = "--no-bpm --no-batched --no-stress"
 EVAL_KERNEL = "tonal_eval_kernel"
+PLAIN_KERNEL = "tonal_plain_kernel"
 CALIB_BYTES = 1 << 30
 
 
@@ -102,8 +103,8 @@ def read_traffic():
 def summarize_pmc(fetch_dir, write_dir):
     """Turns two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) of
     `bench.py --steps 1 --warmup 0 --no-fp64 --no-bpm --no-batched --no-stress --oracle-obs 1
-    --copy-calibration` into the traffic summary, scaling each counter by its reading for the
-    calibration copy."""
+    --copy-calibration` into the traffic summary (the mixed eval kernel, and the plain FP64
+    kernel), scaling each counter by its reading for the calibration copy."""
     import csv
 
     def load(d, counter):
@@ -123,16 +124,19 @@ def summarize_pmc(fetch_dir, write_dir):
         # the calibration copy is the first copy dispatch of the process (before any staging)
         copy = [v for k, v in rows if "copy" in k.lower()]
         ev = [v for k, v in rows if EVAL_KERNEL in k]
+        pl = [v for k, v in rows if PLAIN_KERNEL in k]
         if not copy or not ev:
             raise SystemExit(f"{name}: calibration copy or eval kernel missing in {d}")
         copy_kib = copy[0]
         scale = (CALIB_BYTES / 1024.0) / copy_kib
         out[name] = dict(copy_kib=copy_kib, scale=scale, eval_kib=float(np.mean(ev)),
-                         eval_bytes=float(np.mean(ev)) * 1024.0 * scale)
+                         eval_bytes=float(np.mean(ev)) * 1024.0 * scale,
+                         plain_kib=float(np.mean(pl)) if pl else None,
+                         plain_bytes=float(np.mean(pl)) * 1024.0 * scale if pl else None)
     total = out["FETCH_SIZE"]["eval_bytes"] + out["WRITE_SIZE"]["eval_bytes"]
     lines = [
         "# HBM traffic of the tonal eval kernel: four-blade rotor (4 x 20,000 elements, 1024 steps,",
-        "# 1024 observers), mixed path, one launch",
+        "# 1024 observers), mixed path, one launch (and the plain FP64 kernel's, one launch)",
         "",
         "Commands (repository root `R`, output directory `OUT`, on the GPU host, one counter per run):",
         "",
@@ -155,6 +159,10 @@ def summarize_pmc(fetch_dir, write_dir):
         lines.append(f"| {name} | {o['copy_kib']:.0f} | {o['scale'] * 1024:.4f} | {o['eval_kib']:.0f} | "
                      f"{o['eval_bytes']:.4g} |")
     lines += ["", f"traffic_bytes_per_launch: {total:.6g}", ""]
+    if out["FETCH_SIZE"]["plain_bytes"] is not None and out["WRITE_SIZE"]["plain_bytes"] is not None:
+        pf, pw = out["FETCH_SIZE"]["plain_bytes"], out["WRITE_SIZE"]["plain_bytes"]
+        lines += [f"Plain FP64 kernel (`{PLAIN_KERNEL}`), same scale: FETCH {pf:.4g} bytes, WRITE {pw:.4g} "
+                  f"bytes per launch.", "", f"plain_traffic_bytes_per_launch: {pf + pw:.6g}", ""]
     with open(TRAFFIC_SUMMARY, "w") as f:
         f.write("\n".join(lines))
     print("\n".join(lines))
@@ -174,7 +182,9 @@ COUNTER_PASSES = {
 # kernels the instruction-mix profiles report, and the work one launch does (for per-unit rows)
 COUNTER_KERNELS = {
     "rotor": (("tonal_eval_kernel<true, true, false, 20>", "mixed eval", 80000 * 1024 * 1024 / 64, "wave-eval"),
-              ("tonal_eval_kernel<false, true, false, 20>", "FP64 eval", 80000 * 1024 * 1024 / 64, "wave-eval")),
+              ("tonal_eval_kernel<false, true, false, 20>", "FP64 eval", 80000 * 1024 * 1024 / 64, "wave-eval"),
+              ("tonal_plain_kernel", "plain FP64 kernel (the reference's structure)", 80000 * 1024 * 1024 / 64,
+               "wave-eval")),
     "bpm": (("bpm_obs_kernel<17>", "broadband observer kernel", 240 * 360 * 1024 * 34 / 64, "wave band-eval"),),
 }
 
@@ -202,7 +212,7 @@ def summarize_counters(dirs, workload="rotor"):
         summ = "python3 $R/bench.py --workload bpm --summarize-counters $OUT/*"
     else:
         title = ("tonal eval kernels, four-blade rotor (4 x 20,000 elements, 1024 steps, 1024 observers), "
-                 "mixed and FP64 paths, one launch each")
+                 "mixed and FP64 paths and the plain FP64 kernel, one launch each")
         run = f"python3 $R/bench.py --steps 1 --warmup 0 {TONAL_ONLY} --oracle-obs 1"
         out = os.path.join(ROOT, "profiles", f"tonal_eval_instruction_mix_{TONAL_PROFILE}.md")
         summ = "python3 $R/bench.py --summarize-counters $OUT/*"
@@ -277,7 +287,8 @@ def summarize_stats(d, workload="rotor"):
                 "# 1024 observers), mixed and FP64 passes"]
         cmd = [f"    python3 $R/bench.py --steps 3 --warmup 1 {TONAL_ONLY} --oracle-obs 2",
                "python3 $R/bench.py --summarize-stats $OUT/prof"]
-        note_ = ["`tonal_eval_kernel<true, ...>` is the mixed path, `<false, ...>` the FP64 path.", ""]
+        note_ = ["`tonal_eval_kernel<true, ...>` is the mixed path, `<false, ...>` the FP64 path, "
+                 "`tonal_plain_kernel` the plain FP64 path (the reference's kernel structure, one pass).", ""]
         out = os.path.join(ROOT, "profiles", f"tonal_kernel_stats_{TONAL_PROFILE}.md")
     lines = head + ["", "Command (repository root `R`, output directory `OUT`, on the GPU host):", "", "```",
                     "cd /tmp && export TMPDIR=/tmp",
