@@ -420,6 +420,28 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
               return load_words<Attr>(ae + kk);
           };
           Rec a0r = load_rec(k0), a1r;
+          // The window slots of the step to be evaluated.  In the mixed variant without the tail
+          // they are read at the end of the step before it (after that step's writes, which they
+          // may overlap): the step's one wait, at its top, then covers them with the scalar record
+          // loads issued a whole step earlier, and no wait in the middle of the step holds up the
+          // record loads issued at its top.  The other variants, short of registers to keep the
+          // slots across a step, read them at the top of the step, which waits for them (and for
+          // that step's record loads) before the window update.
+          constexpr bool AHEAD = MIXED && !TAIL;
+          R2* wp;
+          R2 v0, v1, v2, v3;
+          auto read_window = [&](int ic) __attribute__((always_inline)) {
+            // (the footprint's first slot ic - 1 - base clamped into [0, W - 4], as a byte
+            // address: the clamp bounds are the lane's addresses of slots 0 and W - 4; the FP64
+            // path, short of registers, clamps the slot index instead)
+            wp = MIXED ? (R2*)((char*)win + med3i(ic * (64 * (int)sizeof(R2)) + wofs, wlo, whi))
+                       : &win[min(max(ic - 1 - base, 0), W - 4) * 64 + lane];
+            v0 = wp[0];
+            v1 = wp[64];
+            v2 = wp[128];
+            v3 = wp[192];
+          };
+          if constexpr (AHEAD) read_window(s0.ic);
           auto step = [&](int kk, bool wrap, Step& cur, Step& prv, const PosRec& pn, PosRec& pnn,
                           const Rec& ac, Rec& an) __attribute__((always_inline)) {
             const int kn = (kk + 1 < nT) ? kk + 1 : 0;
@@ -430,16 +452,10 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             __builtin_amdgcn_s_waitcnt(0xc07f);
             pnn = load_words<PosRec>(pe + knn);
             an = load_rec(kn);
-            // the footprint is known before the solve: start its window reads now
-            // (the footprint's first slot ic - 1 - base clamped into [0, W - 4], as a byte
-            // address: the clamp bounds are the lane's addresses of slots 0 and W - 4)
-            // (the FP64 path, short of registers, clamps the slot index instead)
             const int ic = cur.ic;
-            R2* wp = MIXED ? (R2*)((char*)win + med3i(ic * (64 * (int)sizeof(R2)) + wofs, wlo, whi))
-                           : &win[min(max(ic - 1 - base, 0), W - 4) * 64 + lane];
-            R2 v0 = wp[0], v1 = wp[64], v2 = wp[128], v3 = wp[192];
-            // keep the record loads and window reads at the top of the step: their latency then
-            // overlaps the solve and the integrand instead of the window update
+            if constexpr (!AHEAD) read_window(ic);
+            // keep the record loads (and the window reads) at the top of the step: their latency
+            // then overlaps the solve and the integrand
             __builtin_amdgcn_sched_barrier(0);
             // observer time of the next step
             double dln;
@@ -515,6 +531,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             prv.om = omn;
             diff(pn, &dxc, &dyc, &dzc);
             delc = (R)dln;
+            if constexpr (AHEAD) read_window(prv.ic);  // the next step's (harmless after a pass's last)
           };
           // Steps run in pairs with the state and record buffers in alternating roles (A is the
           // first step's current state, B its next), so nothing is copied between steps.  An odd

@@ -268,8 +268,9 @@ def summarize_counters(dirs, workload="rotor"):
 def summarize_stats(d, workload="rotor"):
     """Kernel time table of one rocprofv3 --kernel-trace --stats run of
     `bench.py --steps 3 --warmup 1 --no-bpm --no-batched --no-stress --oracle-obs 2` (both tonal
-    precisions), or with
-    workload "bpm" of `bench.py --workload bpm --steps 3`."""
+    precisions and the plain FP64 path), or with workload "bpm" of `bench.py --workload bpm
+    --steps 3`, "stress" of `bench.py --workload stress --steps 2 --oracle-obs 16`, "batched" of
+    `bench.py --workload batched --cases 8 --wave 4 --steps 2`."""
     import csv
     files = glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True)
     if not files:
@@ -282,6 +283,23 @@ def summarize_stats(d, workload="rotor"):
                "python3 $R/bench.py --workload bpm --summarize-stats $OUT/prof"]
         note_ = ["`bpm_obs_kernel<17>` covers the 34 bands in two passes of 17 within one launch per call.", ""]
         out = os.path.join(ROOT, "profiles", f"bpm_kernel_stats_{BPM_PROFILE}.md")
+    elif workload == "stress":
+        head = ["# Kernel statistics: stress case, coaxial 2 x 4 blades x 50,000 elements, 2048 steps,",
+                "# 4096 observers, mixed path (one planning pass and two timed passes)"]
+        cmd = ["    python3 $R/bench.py --workload stress --steps 2 --oracle-obs 16",
+               "python3 $R/bench.py --workload stress --summarize-stats $OUT/prof"]
+        note_ = ["One `tonal_eval_kernel<true, true, false, 20>` launch per pass covers the whole case "
+                 "(3.36e12 evals).", ""]
+        out = os.path.join(ROOT, "profiles", "tonal_kernel_stats_stress_coaxial_8x50000_elements_2048_steps_4096_observers.md")
+    elif workload == "batched":
+        head = ["# Kernel statistics: batched workload, first 8 of the 64 seeded uneven rotor cases, 4 per",
+                "# persistent launch, mixed and FP64 passes, tile shapes per case from the autotuner"]
+        cmd = ["    python3 $R/bench.py --workload batched --cases 8 --wave 4 --steps 2",
+               "python3 $R/bench.py --workload batched --summarize-stats $OUT/prof"]
+        note_ = ["The autotuner's candidate passes run on the first 512 observers of a case and are "
+                 "counted here too; each timed batch pass is one eval launch per kernel variant in use, "
+                 "replayed as a HIP graph.", ""]
+        out = os.path.join(ROOT, "profiles", "tonal_kernel_stats_batched_first_8_seeded_cases.md")
     else:
         head = ["# Kernel statistics: tonal path, four-blade rotor (4 x 20,000 elements, 1024 steps,",
                 "# 1024 observers), mixed and FP64 passes"]
@@ -429,8 +447,29 @@ def leg_batched(acum, O, P, ctx, host, ncases, per_launch, oracle_obs, steps=2, 
     acc = []
     windows = {}
     cpu_evals, cpu_s, cpu_obs = 0, 0.0, 0
-    for w0 in range(0, ncases, per_launch):
-        idx = list(range(w0, min(ncases, w0 + per_launch)))
+    # Cases go into persistent launches of at most `per_launch`, in order, each launch holding what
+    # fits a device-memory budget (the 64 cases need ~446 GB of element data, so they run in
+    # groups): per case, records x (72 B uploaded arrays + 64 B positions + 64 B mixed attributes,
+    # + 128 B FP64 attributes) and its partial rows (at most a quarter of its records or 8 GiB).
+    def dev_bytes(i):
+        p = acum.batch_params(seed, i)
+        rec = p.nBlades * (2 if p.coaxial else 1) * p.nChord * p.nSpan * 2 * p.nTime
+        per = 200 + (128 if fp64 else 0)
+        slab = max(8 << 30, rec * 32) + (max(8 << 30, rec * 48) if fp64 else 0)
+        return rec * per + slab
+    budget = 260e9
+    groups, cur, used = [], [], 0.0
+    for i in range(ncases):
+        b = dev_bytes(i)
+        if cur and (len(cur) == per_launch or used + b > budget):
+            groups.append(cur)
+            cur, used = [], 0.0
+        cur.append(i)
+        used += b
+    if cur:
+        groups.append(cur)
+    for idx in groups:
+        w0 = idx[0]
         t0 = time.time()
         runs = [acum.Run.synth(acum.batch_params(seed, i)).preprocess() for i in idx]
         cases = [r.tonal_case(acum.MIXED) for r in runs]
@@ -487,8 +526,9 @@ def leg_batched(acum, O, P, ctx, host, ncases, per_launch, oracle_obs, steps=2, 
         del runs, cases, batch_out
     value = tot_evals / (mixed_ms * 1e-3) / 1e9
     peak = mixed_peak()
-    out = dict(workload=f"first {ncases} of the 64 seeded uneven rotor cases, {per_launch} per persistent "
-                        f"launch, tile shapes per case from the autotuner, HIP graph replay",
+    out = dict(workload=f"first {ncases} of the 64 seeded uneven rotor cases, at most {per_launch} per persistent "
+                        f"launch ({len(groups)} launches, grouped within a device-memory budget), tile shapes "
+                        f"per case from the autotuner, HIP graph replay",
                evals=tot_evals, value=round(value, 3), unit="G src-obs-time evals/s",
                gpu_ms_per_pass=round(mixed_ms, 2),
                roofline={"bound": "valu", "kernel": "tonal_eval_kernel (all launches of a pass)",
@@ -717,9 +757,12 @@ def main():
         note("timing the FP64 path")
         s64, e64, r64, t64 = timed(acum.FP64, max(1, min(args.steps, 3)), 1)
         plain = ctx.fetch(h, 0, case.nTime, case.nObs)
+        t64s = ctx.case_tuning(h, 0, acum.FP64)
         fp64 = dict(value=round(evals / s64 / 1e9, 3), ms_per_step=round(s64 * 1e3, 3),
                     eval_ms=round(e64, 3), reduce_ms=round(r64, 4), partials=t64.n_partials,
-                    roofline_frac=round(FLOP_PER_EVAL * evals / (e64 * 1e-3) / PEAK_FP64, 4))
+                    roofline_frac=round(FLOP_PER_EVAL * evals / (e64 * 1e-3) / PEAK_FP64, 4),
+                    tile={"window": t64s.window, "max_spread": t64s.max_spread, "max_tile": t64s.max_tile,
+                          "motion_permille": t64s.motion_permille})
     ctx.release(h)
     plain_fp64 = plain_out = None
     if not args.no_plain:
