@@ -34,9 +34,10 @@
 namespace acum {
 
 // Records are read with wave-uniform addresses.  Through constant-address-space pointers the
-// reads are scalar loads into SGPRs (positions, and the mixed path's attributes); through
-// global-address-space pointers the compiler may also use vector loads, which keeps the larger
-// FP64 attribute records out of the SGPR file.
+// reads are scalar loads into SGPRs (positions, the mixed path's attribute scalars and the FP64
+// path's three attribute scalars); through global-address-space pointers they are vector loads
+// (the mixed path's per-lane attribute row, the FP64 path's attribute rows, which would not fit
+// the SGPR file, and the next element's pass-start positions).
 template <typename T>
 using cptr = const __attribute__((address_space(1))) T*;
 template <typename T>
