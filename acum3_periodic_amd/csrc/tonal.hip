@@ -21,8 +21,8 @@
 // Results are deterministic: no operation depends on scheduling order.
 //
 // Precision: MIXED computes observer time, bin index and the bin time deltas in FP64 and the
-// geometry, Doppler factor, integrands and tent weights in FP32 (FP32 window and slab); PLAIN
-// does everything in FP64.  The FP64 observer-time/bin code is the canonical form the CPU
+// geometry, Doppler factor, integrands and tent weights in FP32 (FP32 window and slab); the FP64
+// variant does everything in FP64 (tonal_plain.hip keeps the reference's structure in FP64).  The FP64 observer-time/bin code is the canonical form the CPU
 // oracle uses, compiled without contraction, so bin indices agree bit for bit.
 #include <hip/hip_runtime.h>
 #include <cstddef>

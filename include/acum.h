@@ -41,7 +41,8 @@ enum {
 
 /* Arithmetic of the tonal path. */
 enum {
-  ACUM_FP64 = 0,  /* every operation in FP64 ("plain" path)                                  */
+  ACUM_FP64 = 0,  /* every operation in FP64, restructured like the mixed path (the reference's
+                     own structure in FP64 is acum_tonal_plain)                                */
   ACUM_MIXED = 1  /* FP64 observer time, bin index and bin deltas; FP32 geometry, Doppler and
                      integrands; FP32 per-tile partials summed in FP64 with Kahan compensation */
 };
