@@ -52,4 +52,33 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return y;
 }
 
+// The FP64 tonal path's record products (tonal.hip).  From the lane's field f of a 16-double
+// record held one field per lane of each 16-lane row (lane l holds field l % 16; fields 0-11 the
+// rows v, vd, l, ld, 12-14 aM1, aLM, cT) and r = (rx, ry, rz): rm -= v.r (rm enters as the delay),
+// nVd = aM1 - vd.r, dL += l.r (dL enters as zero), nLd = aLM - ld.r and cT, each product
+// accumulated x, then y, then z, one rounding per FMA.  `s_nop 4` gives the wait states a DPP
+// operand needs after a VALU write of its register or of EXEC.
+__device__ __forceinline__ void rec_products64(double f, double rx, double ry, double rz, double& rm,
+                                               double& nVd, double& dL, double& nLd, double& cT) {
+  __asm__(
+      "s_nop 4\n\t"
+      "v_mov_b64_dpp %1, %5 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+      "v_mov_b64_dpp %3, %5 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_mov_b64_dpp %4, %5 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%5, %6 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%5, %6 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, %5, %6 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, -%5, %6 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%5, %7 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%5, %7 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, %5, %7 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, -%5, %7 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %0, -%5, %8 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, -%5, %8 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, %5, %8 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, -%5, %8 row_newbcast:11 row_mask:0xf bank_mask:0xf"
+      : "+v"(rm), "=&v"(nVd), "+v"(dL), "=&v"(nLd), "=&v"(cT)
+      : "v"(f), "v"(rx), "v"(ry), "v"(rz));
+}
+
 }  // namespace acum

@@ -34,10 +34,9 @@
 namespace acum {
 
 // Records are read with wave-uniform addresses.  Through constant-address-space pointers the
-// reads are scalar loads into SGPRs (positions, the mixed path's attribute scalars and the FP64
-// path's three attribute scalars); through global-address-space pointers they are vector loads
-// (the mixed path's per-lane attribute row, the FP64 path's attribute rows, which would not fit
-// the SGPR file, and the next element's pass-start positions).
+// reads are scalar loads into SGPRs (positions and the mixed path's attribute scalars); through
+// global-address-space pointers they are vector loads (the mixed path's per-lane attribute row,
+// the FP64 path's per-lane attribute field and the next element's pass-start positions).
 template <typename T>
 using cptr = const __attribute__((address_space(1))) T*;
 template <typename T>
@@ -177,33 +176,32 @@ __device__ __forceinline__ R integrand_tail(R aM1, R aLM, R cT, R del, R dV, R d
   return rm;
 }
 
-// FP64 path: what a step reads of its attribute record -- the four rows (96 bytes) through
-// vector loads and the three scalars (24 bytes) through scalar loads.  The rows' wave-uniform
-// vector loads bound this path (each is a 64-lane memory instruction): halving their bytes made
-// it 14% faster in a timing test, and the scalars need only 6 SGPRs per record.
-struct Rows64 {
-  double v[3], vd[3], l[3], ld[3];
-};
-struct Tail64 {
-  double aM1, aLM, cT;
-};
-struct F64Rec {
-  Rows64 r;
-  Tail64 t;
-};
-static_assert(offsetof(AttrT<double>, aM1) == sizeof(Rows64), "FP64 attribute record layout");
+// FP64 path: what a step reads of its attribute record.  The sixteen doubles of the record
+// (v[3], vd[3], l[3], ld[3], aM1, aLM, cT, unused) reach the lanes through one 8-byte vector load
+// per lane: lane l holds field l % 16, so each 16-lane row of the wave holds the whole record, and
+// a step reads field j from lane j of the lane's own row by DPP (row_newbcast:j, the one DPP
+// control of FP64 instructions), on the operand of v_fmac_f64 for the record products and by
+// v_mov_b64 for the three scalars (rec_products64).  Read as rows through wave-uniform vector
+// loads (six 16-byte loads per step, each a 64-lane memory instruction) the record bound this
+// path (this form is 7% faster), and through scalar loads it would not fit the SGPR file.
+static_assert(offsetof(AttrT<double>, aM1) == 12 * sizeof(double), "FP64 attribute record layout");
+static_assert(sizeof(AttrT<double>) == 16 * sizeof(double), "FP64 attribute record layout");
 
+// integrand_tail's arithmetic with rm = del - dV out of the products, and -(dVd - aM1) and
+// -(dLd - aLM) too, whose signs fold into the instructions that use them
 template <bool MOVING>
-__device__ __forceinline__ double integrand(const F64Rec& q, double dx, double dy, double dz, double del,
-                                            double KB, double vx, double vy, double vz, double* pth,
-                                            double* plo) {
+__device__ __forceinline__ double integrand(double f, double dx, double dy, double dz, double del, double KB,
+                                            double vx, double vy, double vz, double* pth, double* plo) {
   double rx, ry, rz;
   radiation_vector<double, MOVING>(dx, dy, dz, del, vx, vy, vz, &rx, &ry, &rz);
-  const double dV = fmaT(q.r.v[2], rz, fmaT(q.r.v[1], ry, q.r.v[0] * rx));
-  const double dVd = fmaT(q.r.vd[2], rz, fmaT(q.r.vd[1], ry, q.r.vd[0] * rx));
-  const double dL = fmaT(q.r.l[2], rz, fmaT(q.r.l[1], ry, q.r.l[0] * rx));
-  const double dLd = fmaT(q.r.ld[2], rz, fmaT(q.r.ld[1], ry, q.r.ld[0] * rx));
-  return integrand_tail<double>(q.t.aM1, q.t.aLM, q.t.cT, del, dV, dVd, dL, dLd, KB, pth, plo);
+  double rm = del, nVd, dL = 0.0, nLd, cT;
+  rec_products64(f, rx, ry, rz, rm, nVd, dL, nLd, cT);
+  const double g = rcpT(rm);
+  const double g2 = g * g;
+  const double tn = g * nVd;  // -g w
+  *pth = (cT * g2) * fmaT(-del, tn, -KB);
+  *plo = g2 * -fmaT(dL, tn, nLd);
+  return rm;
 }
 
 // Mixed path: the four record products of 64 observers by the matrix cores.  With
@@ -299,13 +297,11 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
   using R = typename P::R;
   using R2 = typename P::R2;
   using Attr = AttrT<R>;
-  // what a step reads of its attribute record: the rows through vector loads and the scalars
-  // through scalar loads (FP64), or the scalar fields and the lane's row for the matrix-core
-  // products (mixed)
-  using Rec = std::conditional_t<MIXED, MixRec, F64Rec>;
+  // what a step reads of its attribute record: the lane's field, broadcast over its row (FP64),
+  // or the scalar fields and the lane's row for the matrix-core products (mixed)
+  using Rec = std::conditional_t<MIXED, MixRec, double>;
   // positions always through scalar loads (the FP64 path's went through vector loads, 25% slower
-  // there: a wave-uniform record has no use for 64 lanes of vector addressing); the FP64 path's
-  // 128-byte attribute records stay in vector registers, out of the SGPR file
+  // there: a wave-uniform record has no use for 64 lanes of vector addressing)
   using PosPtr = kptr<PosRec>;
   using AttrPtr = typename P::template ptr<Attr>;
   // the mixed path's weights are FP32: one residual correction of the observer time's root
@@ -436,9 +432,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             if constexpr (MIXED)
               return load_mixrec((const Attr*)(uintptr_t)(ae + kk), lane);
             else
-              return F64Rec{load_words<Rows64>((cptr<Rows64>)(uintptr_t)(ae + kk)),
-                            load_words<Tail64>((kptr<Tail64>)(uintptr_t)((const char*)(uintptr_t)(ae + kk) +
-                                                                         sizeof(Rows64)))};
+              return ((cptr<double>)(uintptr_t)(ae + kk))[lane & 15];
           };
           Rec a0r = load_rec(k0), a1r;
           // The window slots of the step to be evaluated.  In the variants without the tail they

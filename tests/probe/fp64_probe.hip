@@ -65,6 +65,37 @@ extern "C" int probe_mfma4x4(const float* a, const float* b, float* out) {
   return (int)e;
 }
 
+// One wave of rec_products64: lane l holds field f[l] (each 16-lane row its own record) and its
+// own r = (r[l], r[64 + l], r[128 + l]) and delay d[l]; out[l][0..4] = rm, nVd, dL, nLd, cT of
+// lane l.  The test checks the row broadcast the FP64 tonal kernel relies on.
+__global__ void rec64_kernel(const double* __restrict__ f, const double* __restrict__ r,
+                             const double* __restrict__ d, double* __restrict__ out) {
+  const int l = threadIdx.x;
+  double rm = d[l], nVd, dL = 0.0, nLd, cT;
+  acum::rec_products64(f[l], r[l], r[64 + l], r[128 + l], rm, nVd, dL, nLd, cT);
+  out[5 * l] = rm;
+  out[5 * l + 1] = nVd;
+  out[5 * l + 2] = dL;
+  out[5 * l + 3] = nLd;
+  out[5 * l + 4] = cT;
+}
+
+// f: 64 doubles, r: 3 x 64, d: 64; out: 64 x 5.
+extern "C" int probe_rec64(const double* f, const double* r, const double* d, double* out) {
+  double* dv = nullptr;  // f | r | d | out
+  hipError_t e = hipMalloc(&dv, (64 + 192 + 64 + 320) * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(dv, f, 64 * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dv + 64, r, 192 * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dv + 256, d, 64 * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(rec64_kernel, dim3(1), dim3(64), 0, 0, dv, dv + 64, dv + 256, dv + 320);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, dv + 320, 320 * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(dv);
+  return (int)e;
+}
+
 // x: host array of n positive doubles; s, r: host outputs (sqrt, reciprocal).  Returns 0 on
 // success, a HIP error code otherwise.
 extern "C" int probe_fp64(const double* x, double* s, double* r, int64_t n) {

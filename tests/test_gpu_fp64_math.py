@@ -68,3 +68,26 @@ def test_mfma_4x4x1_lane_layout():
         for v in range(4):
             want[l, v] = sum(a[k, 4 * blk + v] * b[k, 4 * blk + j] for k in range(3))
     assert np.array_equal(out, want), f"layout differs:\n{out[:8]}\n{want[:8]}"
+
+
+def test_rec_products64_row_broadcast():
+    """rec_products64 as the FP64 tonal kernel uses it: lane l holds field l % 16 of its 16-lane
+    row's record, and each lane gets its row's record dotted into its own r: rm = d - v.r,
+    nVd = aM1 - vd.r, dL = l.r, nLd = aLM - ld.r and cT (fields 12, 13, 14)."""
+    lib = C.CDLL(PROBE)
+    dp = C.POINTER(C.c_double)
+    lib.probe_rec64.argtypes = [dp, dp, dp, dp]
+    lib.probe_rec64.restype = C.c_int
+    rng = np.random.default_rng(7)
+    f = rng.integers(-9, 10, size=64).astype(np.float64)  # small integers: every sum exact
+    r = rng.integers(-9, 10, size=(3, 64)).astype(np.float64)
+    d = rng.integers(-99, 100, size=64).astype(np.float64)
+    out = np.zeros((64, 5))
+    assert lib.probe_rec64(f.ctypes.data_as(dp), r.ctypes.data_as(dp), d.ctypes.data_as(dp),
+                           out.ctypes.data_as(dp)) == 0
+    want = np.zeros((64, 5))
+    for l in range(64):
+        rec = f[16 * (l // 16):16 * (l // 16) + 16]
+        dot = lambda j: sum(rec[3 * j + c] * r[c, l] for c in range(3))
+        want[l] = (d[l] - dot(0), rec[12] - dot(1), dot(2), rec[13] - dot(3), rec[14])
+    assert np.array_equal(out, want), f"broadcast differs:\n{out[:4]}\n{want[:4]}"
