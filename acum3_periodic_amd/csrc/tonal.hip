@@ -309,6 +309,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
   // win[slot][lane] = (thickness, loading) partial of one observer-time bin
   __shared__ R2 win[W * 64];
   const int lane = threadIdx.x;
+  [[maybe_unused]] const int loff = (lane & 15) * (int)sizeof(double);  // FP64 record field of the lane
   // the lane's byte addresses of window slots 0 and W - 4 (a footprint's first slot lies between)
   const int wlo = lane * (int)sizeof(R2), whi = ((W - 4) * 64 + lane) * (int)sizeof(R2);
   for (int s = 0; s < W; s++) win[s * 64 + lane] = R2{(R)0, (R)0};
@@ -347,15 +348,18 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
     const double dnT = (double)nT;
     // obs - src of the radiation vector: in FP32 from the parsed floats (mixed; the same bits as
     // the FP64 difference rounded to float), in FP64 as o + h2 m (FP64 path)
+    // (h2 pinned in VGPRs: an FMA may read one scalar operand, and m comes through scalar loads)
+    double h2 = k.h2;
+    if constexpr (!MIXED) __asm__("" : "+v"(h2));
     auto diff = [&](const PosRec& p, R* dx, R* dy, R* dz) __attribute__((always_inline)) {
       if constexpr (MIXED) {
         *dx = ofx - p.x;
         *dy = ofy - p.y;
         *dz = ofz - p.z;
       } else {
-        *dx = __builtin_fma(p.mx, k.h2, L.ox);
-        *dy = __builtin_fma(p.my, k.h2, L.oy);
-        *dz = __builtin_fma(p.mz, k.h2, L.oz);
+        *dx = __builtin_fma(p.mx, h2, L.ox);
+        *dy = __builtin_fma(p.my, h2, L.oy);
+        *dz = __builtin_fma(p.mz, h2, L.oz);
       }
     };
     for (int ti = item.tile0; ti < item.tile0 + item.ntile; ti++) {
@@ -385,10 +389,17 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
         int wofs = (lane - 64 * (base + 1)) * (int)sizeof(R2);
         // kept opaque, so that each step forms the address in one shift-add (the compiler would
         // otherwise rebuild it from its parts with an extra instruction per step)
-        if constexpr (MIXED) __asm__("" : "+v"(wofs));
+        __asm__("" : "+v"(wofs));
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
           const PosPtr pe = pos + e * nT;
           const AttrPtr ae = attr + e * nT;
+          // FP64 path: the element's attribute records as a buffer, so that a step's load takes
+          // the lane's field offset in a VGPR and the step's record offset in an SGPR (a global
+          // load would add a 64-bit vector address every step); dword 3 is gfx9's raw-buffer word
+          [[maybe_unused]] __amdgpu_buffer_rsrc_t arec;
+          if constexpr (!MIXED)
+            arec = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)ae, (short)0, nT * (int)sizeof(Attr),
+                                                     0x00020000);
           const PosSolve pm_e = nm;
           const PosRec pc_e = nc;
           if (e + 1 < tl.e0 + tl.ne) {
@@ -432,7 +443,12 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             if constexpr (MIXED)
               return load_mixrec((const Attr*)(uintptr_t)(ae + kk), lane);
             else
-              return ((cptr<double>)(uintptr_t)(ae + kk))[lane & 15];
+            {
+              const auto v = __builtin_amdgcn_raw_buffer_load_b64(arec, loff, kk * (int)sizeof(Attr), 0);
+              double f;
+              __builtin_memcpy(&f, &v, sizeof f);
+              return f;
+            }
           };
           Rec a0r = load_rec(k0), a1r;
           // The window slots of the step to be evaluated.  In the variants without the tail they
@@ -449,7 +465,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             // (the footprint's first slot ic - 1 - base clamped into [0, W - 4], as a byte
             // address: the clamp bounds are the lane's addresses of slots 0 and W - 4; the FP64
             // path, short of registers, clamps the slot index instead)
-            wp = MIXED ? (R2*)((char*)win + med3i(ic * (64 * (int)sizeof(R2)) + wofs, wlo, whi))
+            wp = true ? (R2*)((char*)win + med3i(ic * (64 * (int)sizeof(R2)) + wofs, wlo, whi))
                        : &win[min(max(ic - 1 - base, 0), W - 4) * 64 + lane];
             v0 = wp[0];
             v1 = wp[64];
@@ -492,7 +508,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             const R frm = fr - (R)1;
             // (1 - idp carried from the step before in the mixed path; the FP64 path, short of
             // registers, forms it again)
-            const R w0 = clamp01(fmaT(-fr, idp, MIXED ? cur.om : (R)1 - idp));  // bin ic-1: 1 - (fr + 1) / dp
+            const R w0 = clamp01(fmaT(-fr, idp, cur.om));  // bin ic-1: 1 - (fr + 1) / dp
             const R w1 = clamp01(fmaT(-fr, idp, (R)1));        // bin ic:   1 - fr / dp
             const R w2 = clamp01(fmaT(frm, idn, (R)1));        // bin ic+1: 1 - (1 - fr) / dn
             const R w3 = clamp01(fmaT(frm, idn, omn));         // bin ic+2: 1 - (2 - fr) / dn
