@@ -463,10 +463,8 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
           R2 v0, v1, v2, v3;
           auto read_window = [&](int ic) __attribute__((always_inline)) {
             // (the footprint's first slot ic - 1 - base clamped into [0, W - 4], as a byte
-            // address: the clamp bounds are the lane's addresses of slots 0 and W - 4; the FP64
-            // path, short of registers, clamps the slot index instead)
-            wp = true ? (R2*)((char*)win + med3i(ic * (64 * (int)sizeof(R2)) + wofs, wlo, whi))
-                       : &win[min(max(ic - 1 - base, 0), W - 4) * 64 + lane];
+            // address: the clamp bounds are the lane's addresses of slots 0 and W - 4)
+            wp = (R2*)((char*)win + med3i(ic * (64 * (int)sizeof(R2)) + wofs, wlo, whi));
             v0 = wp[0];
             v1 = wp[64];
             v2 = wp[128];
@@ -506,8 +504,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             const R idp = cur.id, idn = rcpT(dn);
             const R omn = (R)1 - idn;
             const R frm = fr - (R)1;
-            // (1 - idp carried from the step before in the mixed path; the FP64 path, short of
-            // registers, forms it again)
+            // (1 - idp carried from the step before)
             const R w0 = clamp01(fmaT(-fr, idp, cur.om));  // bin ic-1: 1 - (fr + 1) / dp
             const R w1 = clamp01(fmaT(-fr, idp, (R)1));        // bin ic:   1 - fr / dp
             const R w2 = clamp01(fmaT(frm, idn, (R)1));        // bin ic+1: 1 - (1 - fr) / dn
