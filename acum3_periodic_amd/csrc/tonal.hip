@@ -276,7 +276,7 @@ __device__ __forceinline__ int med3i(int x, int lo, int hi) {
 // zero: a weight is positive exactly when its bin lies inside the tent, so the clamp selects the
 // tent's bins (up to the rounding of the weight itself, below 1e-6 in the mixed path).  Sides
 // longer than two bins go to a wave-uniform tail that counts their bins from the FP64 times.
-constexpr int kFlush = 5;  // window slots per round of the FP64 path's slab update
+constexpr int kFlush = 10;  // window slots per round of the FP64 path's slab update
 
 // Waves per SIMD that the LDS window allows (160 KB per CU, four SIMDs, one wave per workgroup):
 // four for the mixed path's float2 window at W = 20, two for the FP64 path's double2 one.  The
