@@ -159,6 +159,18 @@ def test_errors_are_returned(ctx):
             with pytest.raises(acum.AcumError) as e:
                 ctx.tonal(c)
             assert e.value.code == acum.ACUM_ERR_NUMERIC
+    # empty and degenerate shapes are argument errors, caught before any launch
+    for field, value in (("nObs", 0), ("nTime", 3), ("nSurf", 0)):
+        run, c = small_case(1, acum.MIXED)
+        setattr(c, field, value)
+        with pytest.raises(acum.AcumError) as e:
+            ctx.tonal(c)
+        assert e.value.code == acum.ACUM_ERR_ARG, field
+    run, c = small_case(1, acum.MIXED)
+    c.E[0] = 0  # a surface with no elements
+    with pytest.raises(acum.AcumError) as e:
+        ctx.tonal(c)
+    assert e.value.code == acum.ACUM_ERR_ARG
     run, c = small_case(1, acum.FP64)
     ctx.tonal(c)  # the context keeps working after those errors
 
