@@ -29,6 +29,9 @@ struct PosRec {
 struct PosSolve {
   double mx, my, mz, q, wp, jt;
 };
+// An element's attributes at one step.  The FP64 records are stored in this order; the FP32 ones
+// with each row followed by one scalar, {v, aM1, vd, aLM, l, cT, ld, 0}, for the mixed path's
+// 16-byte row loads.
 template <typename R>
 struct AttrT {
   R v[3], vd[3], l[3], ld[3];

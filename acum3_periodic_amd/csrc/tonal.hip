@@ -33,10 +33,10 @@
 
 namespace acum {
 
-// Records are read with wave-uniform addresses.  Through constant-address-space pointers the
-// reads are scalar loads into SGPRs (positions and the mixed path's attribute scalars); through
-// global-address-space pointers they are vector loads (the mixed path's per-lane attribute row,
-// the FP64 path's per-lane attribute field and the next element's pass-start positions).
+// Positions are read with wave-uniform addresses through constant-address-space pointers, as
+// scalar loads into SGPRs; through global-address-space pointers they are vector loads (the next
+// element's pass-start positions).  Attribute records reach the lanes through per-lane buffer
+// loads (the mixed path's row per lane, the FP64 path's field per lane) and DPP broadcasts.
 template <typename T>
 using cptr = const __attribute__((address_space(1))) T*;
 template <typename T>
@@ -209,25 +209,19 @@ __device__ __forceinline__ double integrand(double f, double dx, double dy, doub
 // A and column j of its B and receives column j of block b's result: so lane l supplies row
 // l % 4 of the 4 x 3 record matrix (v, vd, l, ld) for A, its observer's r_vec component for B,
 // and after three accumulating products (x, y, z) holds dV, dVd, dL, dLd of its own observer.
-// The record rows reach the lanes by one 12-byte vector load per lane from the attribute record
-// (rows at float offsets 0, 3, 6, 9); the scalar fields come through a 16-byte scalar load.
+// The record reaches the lanes by one 16-byte vector load per lane: the mixed attribute record
+// stores each row with one scalar field after it ({v, aM1}, {vd, aLM}, {l, cT}, {ld, 0}), lane l
+// loads row l % 4, and the integrand reads aM1, aLM and cT from lanes 0, 1 and 2 of its block by
+// DPP (quad_perm) on the instructions that use them.
 typedef float v4f __attribute__((ext_vector_type(4)));
-struct RecRow {
-  float x, y, z;
-};
-struct RecTail {
-  float aM1, aLM, cT, unused;
-};
 struct MixRec {
-  RecTail t;
-  RecRow row;
+  float x, y, z, w;  // row l % 4 of the record and the scalar field stored after it
 };
-__device__ __forceinline__ MixRec load_mixrec(const AttrT<float>* rec, int lane) {
-  static_assert(sizeof(AttrT<float>) == 64, "attribute record layout");
-  MixRec m;
-  m.t = load_words<RecTail>((kptr<RecTail>)(uintptr_t)((const char*)rec + 48));
-  m.row = load_words<RecRow>((cptr<RecRow>)(uintptr_t)((const float*)rec + 3 * (lane & 3)));
-  return m;
+static_assert(sizeof(AttrT<float>) == 4 * sizeof(MixRec), "mixed attribute record layout");
+// field w of lane J of the lane's block of four (J = 0, 1, 2: aM1, aLM, cT)
+template <int J>
+__device__ __forceinline__ float quad_field(float w) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(w), J * 0x55, 0xf, 0xf, true));
 }
 
 template <bool MOVING>
@@ -237,10 +231,11 @@ __device__ __forceinline__ float integrand_mfma(const MixRec& q, float dx, float
   float rx, ry, rz;
   radiation_vector<float, MOVING>(dx, dy, dz, del, vx, vy, vz, &rx, &ry, &rz);
   v4f d = {0.0f, 0.0f, 0.0f, 0.0f};
-  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.x, rx, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.y, ry, d, 0, 0, 0);
-  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.row.z, rz, d, 0, 0, 0);
-  return integrand_tail<float>(q.t.aM1, q.t.aLM, q.t.cT, del, d[0], d[1], d[2], d[3], KB, pth, plo);
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.x, rx, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.y, ry, d, 0, 0, 0);
+  d = __builtin_amdgcn_mfma_f32_4x4x1f32(q.z, rz, d, 0, 0, 0);
+  return integrand_tail<float>(quad_field<0>(q.w), quad_field<1>(q.w), quad_field<2>(q.w), del, d[0], d[1],
+                               d[2], d[3], KB, pth, plo);
 }
 
 // The high word of a finite value read as a signed integer is positive exactly when the value is
@@ -298,7 +293,7 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
   using R2 = typename P::R2;
   using Attr = AttrT<R>;
   // what a step reads of its attribute record: the lane's field, broadcast over its row (FP64),
-  // or the scalar fields and the lane's row for the matrix-core products (mixed)
+  // or the lane's row for the matrix-core products with one scalar field after it (mixed)
   using Rec = std::conditional_t<MIXED, MixRec, double>;
   // positions always through scalar loads (the FP64 path's went through vector loads, 25% slower
   // there: a wave-uniform record has no use for 64 lanes of vector addressing)
@@ -309,7 +304,8 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
   // win[slot][lane] = (thickness, loading) partial of one observer-time bin
   __shared__ R2 win[W * 64];
   const int lane = threadIdx.x;
-  [[maybe_unused]] const int loff = (lane & 15) * (int)sizeof(double);  // FP64 record field of the lane
+  // the lane's byte offset in an attribute record: row lane % 4 (mixed), field lane % 16 (FP64)
+  const int loff = MIXED ? (lane & 3) * 16 : (lane & 15) * (int)sizeof(double);
   // the lane's byte addresses of window slots 0 and W - 4 (a footprint's first slot lies between)
   const int wlo = lane * (int)sizeof(R2), whi = ((W - 4) * 64 + lane) * (int)sizeof(R2);
   for (int s = 0; s < W; s++) win[s * 64 + lane] = R2{(R)0, (R)0};
@@ -393,13 +389,11 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
         for (int64_t e = tl.e0; e < tl.e0 + tl.ne; e++) {
           const PosPtr pe = pos + e * nT;
           const AttrPtr ae = attr + e * nT;
-          // FP64 path: the element's attribute records as a buffer, so that a step's load takes
-          // the lane's field offset in a VGPR and the step's record offset in an SGPR (a global
-          // load would add a 64-bit vector address every step); dword 3 is gfx9's raw-buffer word
-          [[maybe_unused]] __amdgpu_buffer_rsrc_t arec;
-          if constexpr (!MIXED)
-            arec = __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)ae, (short)0, nT * (int)sizeof(Attr),
-                                                     0x00020000);
+          // the element's attribute records as a buffer, so that a step's load takes the lane's
+          // offset in a VGPR and the step's record offset in an SGPR (a global load would add a
+          // 64-bit vector address every step); dword 3 is gfx9's raw-buffer word
+          const __amdgpu_buffer_rsrc_t arec =
+              __builtin_amdgcn_make_buffer_rsrc((void*)(uintptr_t)ae, (short)0, nT * (int)sizeof(Attr), 0x00020000);
           const PosSolve pm_e = nm;
           const PosRec pc_e = nc;
           if (e + 1 < tl.e0 + tl.ne) {
@@ -440,10 +434,12 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
           // of the integrand, each with one load in flight
           PosRec p0 = load_words<PosRec>(pe + kn0), p1;
           auto load_rec = [&](int kk) __attribute__((always_inline)) -> Rec {
-            if constexpr (MIXED)
-              return load_mixrec((const Attr*)(uintptr_t)(ae + kk), lane);
-            else
-            {
+            if constexpr (MIXED) {
+              const auto v = __builtin_amdgcn_raw_buffer_load_b128(arec, loff, kk * (int)sizeof(Attr), 0);
+              MixRec m;
+              __builtin_memcpy(&m, &v, sizeof m);
+              return m;
+            } else {
               const auto v = __builtin_amdgcn_raw_buffer_load_b64(arec, loff, kk * (int)sizeof(Attr), 0);
               double f;
               __builtin_memcpy(&f, &v, sizeof f);
@@ -799,7 +795,14 @@ __global__ void tonal_attr_kernel(const float* __restrict__ X, const float* __re
     o.aLM = (R)(cL * (P * Vn));
     o.cT = (R)(rhoRef * Vn * ds * a0 * D1 * sc);
     o.unused = (R)0;
-    attr[q] = o;
+    if constexpr (sizeof(R) == sizeof(float)) {
+      // the mixed record: each row with one scalar field after it (MixRec)
+      const R w[16] = {o.v[0],  o.v[1],  o.v[2],  o.aM1, o.vd[0], o.vd[1], o.vd[2], o.aLM,
+                       o.l[0],  o.l[1],  o.l[2],  o.cT,  o.ld[0], o.ld[1], o.ld[2], (R)0};
+      __builtin_memcpy(&attr[q], w, sizeof w);
+    } else {
+      attr[q] = o;
+    }
   }
   if (bad) atomicOr(flags, kFlagNumeric);  // non-finite element input
 }
