@@ -475,6 +475,10 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
             // this step's: scalar loads return out of order, so a wait placed after the new loads
             // would wait for them too.  lgkmcnt(0), vmcnt and expcnt unlimited.
             __builtin_amdgcn_s_waitcnt(0xc07f);
+            // (mixed: the pad of the position record read a step ago, used here where it has
+            // arrived, keeps the compiler loading each record whole, in two scalar loads instead
+            // of four; the FP64 path reads only the first 48 bytes)
+            if constexpr (MIXED) __asm__("" ::"s"(pn.pad));
             pnn = load_words<PosRec>(pe + knn);
             an = load_rec(kn);
             const int ic = cur.ic;
