@@ -2,8 +2,9 @@
 //
 // Work decomposition (the reference launches one thread per (step, observer, element) with
 // float atomics on pT/pL/pA, core.cu:18-259, main.cu:194-236):
-//   * one wave = 64 observers (one per lane); element records are wave-uniform scalar loads, so
-//     every record read from HBM serves 64 observers;
+//   * one wave = 64 observers (one per lane); element records are wave-uniform (positions through
+//     scalar loads, attribute records spread over the lanes and broadcast by DPP), so every
+//     record read from HBM serves 64 observers;
 //   * a work item = (case, observer group, group of tiles), taken from one of eight queues (one
 //     per XCD, so the observer groups reading the same records share an L2); a tile is a run of
 //     elements that sit close together; for each tile the wave walks the source steps in passes
@@ -22,8 +23,9 @@
 //
 // Precision: MIXED computes observer time, bin index and the bin time deltas in FP64 and the
 // geometry, Doppler factor, integrands and tent weights in FP32 (FP32 window and slab); the FP64
-// variant does everything in FP64 (tonal_plain.hip keeps the reference's structure in FP64).  The FP64 observer-time/bin code is the canonical form the CPU
-// oracle uses, compiled without contraction, so bin indices agree bit for bit.
+// variant does everything in FP64 (tonal_plain.hip keeps the reference's structure in FP64).
+// The FP64 observer-time/bin code is the canonical form the CPU oracle uses, compiled without
+// contraction, so bin indices agree bit for bit.
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
@@ -449,9 +451,9 @@ void tonal_eval_kernel(const TonalCaseDev* __restrict__ cases,
           Rec a0r = load_rec(k0), a1r;
           // The window slots of the step to be evaluated.  In the variants without the tail they
           // are read at the end of the step before it (after that step's writes, which they may
-          // overlap): the step's one wait, at its top, then covers them with the scalar record
+          // overlap): the step's one wait, at its top, then covers them with the scalar position
           // loads issued a whole step earlier, and no wait in the middle of the step holds up the
-          // record loads issued at its top (mixed -1.7%, FP64 -3.4%).  The variants with the tail,
+          // loads issued at its top (mixed -1.7%, FP64 -3.4%).  The variants with the tail,
           // short of registers to keep the slots across a step, read them at the top of the
           // step, which waits for them (and for that step's record loads) before the update.
           constexpr bool AHEAD = !TAIL;
