@@ -9,6 +9,7 @@ import pytest
 import acum3_periodic_amd as acum
 import oracle as O
 from parity import report
+from tonal_cases import make_case, rotor_surface
 
 pytestmark = pytest.mark.gpu
 
@@ -319,3 +320,32 @@ def test_batch_of_cases_with_different_windows(ctx, tmp_path):
             assert all(np.array_equal(x, y) for x, y in zip(ctx.tonal(c), b))
     finally:
         ctx.tune_cache_clear()
+
+
+@pytest.mark.parametrize("prec", [acum.FP64, acum.MIXED])
+@pytest.mark.parametrize("E,nT,nobs,moving", [(1, 4, 1, False), (1, 4, 1, True), (3, 5, 65, True),
+                                              (2, 7, 63, False)])
+def test_smallest_and_ragged_shapes(ctx, prec, E, nT, nobs, moving):
+    """The smallest case the library accepts (one element, four steps, one observer) and ragged
+    ones (65 and 63 observers: a wave with one or 63 live lanes; odd step counts, periods shorter
+    than the window) against the FP64 oracle, in both precisions."""
+    omega = 150.0
+    dPsi = 2.0 * np.pi / nT
+    dTau = dPsi / omega
+    rng = np.random.default_rng(100 * E + nT)
+    P = rng.uniform(500.0, 2000.0, size=(E, nT))
+    surf = rotor_surface(nT, np.linspace(0.4, 0.9, E), np.zeros(E), omega, dTau, P)
+    i = np.arange(nobs) + 0.5
+    th, ph = np.arccos(1.0 - 2.0 * i / nobs), np.pi * (1.0 + 5.0 ** 0.5) * i
+    rad = 4.0 + 4.0 * (i / nobs)
+    obs = np.stack([rad * np.sin(th) * np.cos(ph), rad * np.sin(th) * np.sin(ph), rad * np.cos(th)], 1)
+    vinf = (20.0, -3.0, -2.0) if moving else (0.0, 0.0, 0.0)
+    c = make_case([surf], obs.astype(np.float32), nT, dPsi, dTau, oM=omega, vinf=vinf, precision=prec)
+    g = ctx.tonal(c)
+    o = O.tonal_f64(c)
+    chans = ("pT", "pL", "pA") if prec == acum.FP64 else ("pA",)
+    bar = REL_L2_FP64 if prec == acum.FP64 else REL_L2_MIXED
+    for ch in chans:
+        k = ("pT", "pL", "pA").index(ch)
+        err = O.rel_l2(g[k], o[k])
+        assert err <= bar, f"E={E} nT={nT} nObs={nobs} {ch}: rel L2 {err:.3e}"
