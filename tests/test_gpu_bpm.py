@@ -26,6 +26,8 @@ VARIANTS = [
     ("laminar_empirical_bl", dict(trip=0, t_del=2)),
     ("laminar_table_bl", dict(trip=0, t_del=1)),
     ("file_bl_42_bands", dict(trip=0, t_del=0, nBands=42)),
+    # 65 observers: a second observer tile with one live lane; 7 steps
+    ("ragged_65_observers", dict(trip=1, t_del=2, nObsAz=13, nObsEl=5, nTime=7)),
 ]
 
 
